@@ -1,0 +1,99 @@
+/*
+ * chns.h - C ABI of the MI355X-native CHNS-2D ETD2RK hot path.
+ *
+ * Drop-in boundary for the time loop of the reference pseudo-spectral
+ * Cahn-Hilliard-Navier-Stokes solver (phynadia/Ternary-CHNS, "CHNS-2D-MPI").
+ * Plain C types only: pointers, sizes, doubles, ints.  Complex arrays are
+ * interleaved (re, im) doubles with the reference layout IDXC(i,j) = j + Ny2*i
+ * (global.h:13), i.e. [Nx][Ny/2+1] complex, x slow.  Real arrays are packed
+ * [Nx][Ny] doubles (the reference's IDXR rows are [Ny+2] with two pad
+ * doubles, global.h:14; callers strip the pad).
+ *
+ * Every call returns 0 on success and a negative CHNS_E* code on failure;
+ * nothing aborts.  chns_last_error() gives the message.  A context owns one
+ * HIP device and one stream; calls on a context are stream-ordered and a
+ * context is not thread-safe.  State lives on the device between calls.
+ */
+#ifndef CHNS_H
+#define CHNS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHNS_OK 0
+#define CHNS_E_ARG (-1)     /* invalid argument or unsupported configuration */
+#define CHNS_E_HIP (-2)     /* HIP runtime error */
+#define CHNS_E_STATE (-3)   /* call out of order (e.g. no state set) */
+
+/* models: macro.h:7-8 (neither BINARY nor TERNARY = Navier-Stokes only) */
+#define CHNS_MODEL_NS 0
+#define CHNS_MODEL_BINARY 1
+#define CHNS_MODEL_TERNARY 2
+
+/* Runtime form of the reference's compile-time parameters (global.h:20-33,
+ * 102-104, 127-134).  `lambda` is the mobility.  epsilon <= 0 selects the
+ * reference default 3*dx.  f_amp/f_k: Kolmogorov forcing f_amp*sin(f_k*y) on
+ * the vorticity (new: the reference forcing.c does not compile). */
+typedef struct {
+  int Nx, Ny, model;
+  double lx, ly, nu, dt;
+  double sigma, epsilon, lambda;
+  double sigma_12, sigma_13, sigma_23, Xlambda;
+  int smooth_filter; /* macro.h:15 SMOOTH_FILTER; must be 0 in this build */
+  double f_amp;
+  int f_k;
+} chns_params;
+
+typedef struct chns_ctx chns_ctx;
+
+/* Fill p with the reference defaults (global.h, macro.h) for `model`. */
+void chns_params_default(chns_params *p, int model);
+
+/* Allocate a context on HIP device `device`.  Replaces main.c:37-194
+ * (MPI/FFTW init, allocation, plans) and kxky() (functions.c:11-32). */
+int chns_create(chns_ctx **out, const chns_params *p, int device);
+void chns_destroy(chns_ctx *ctx);
+const char *chns_last_error(const chns_ctx *ctx);
+
+/* Upload / download the spectral state (host arrays, IDXC layout, 2*Nx*Ny2
+ * doubles each).  cphi1/cphi2 are ignored (may be NULL) when the model has no
+ * such field; for BINARY cphi1 is the reference's cphi.  Replaces the state
+ * arrays cwz, cphi1, cphi2 of global.c:13-49. */
+int chns_set_state(chns_ctx *ctx, const double *cwz, const double *cphi1, const double *cphi2);
+int chns_get_state(chns_ctx *ctx, double *cwz, double *cphi1, double *cphi2);
+
+/* Initial state from packed real [Nx][Ny] fields: dfft + truncation, as
+ * init.c:253-262 (INIT_READ_DATA) and init.c:291-294.  wz may be NULL (zero
+ * vorticity, init.c:78-100). */
+int chns_set_real_state(chns_ctx *ctx, const double *wz, const double *phi1, const double *phi2);
+
+/* Advance nsteps ETD2RK steps on the device: etd2rk() (etd2rk.c:14-224)
+ * called nsteps times, including potential() (chem_potential.c:12-94),
+ * nonlin() (nonlin.c:13-381), surface_tension() (surface_tension.c:13-227),
+ * get_velocity() (velocity.c:3-44), curl() (curl.c:16-31), truncation()
+ * (functions.c:47-83) and the dfft/ifft pairs (fft.c:11-31) inside them.
+ * Returns when the work is queued; chns_sync() waits. */
+int chns_step(chns_ctx *ctx, int nsteps);
+int chns_sync(chns_ctx *ctx);
+
+/* In-place truncation of the phase fields, the state side effect of
+ * spectrum() (spectrum.c:22-27) called at main.c:226,281-284. */
+int chns_truncate_state(chns_ctx *ctx);
+
+/* ---- host-only helpers (no device needed) ---------------------------- */
+
+/* ETD2RK coefficients per mode, IDXC order [Nx][Ny2] each, in the exact
+ * operation order of etd2rk.c:88-127 and :162-203 with the C library exp.
+ * beta/z1c/z2c are left untouched for CHNS_MODEL_NS (may be NULL). */
+int chns_etd_tables(const chns_params *p, double *eta, double *z1w, double *z2w,
+                    double *beta, double *z1c, double *z2c);
+
+/* Dealias extent per ky column j < Ny/2+1: the largest |kx index| kept by
+ * truncation() (functions.c:47-83), -1 if the column is fully truncated. */
+int chns_dealias_extent(const chns_params *p, int *amax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHNS_H */

@@ -1,0 +1,62 @@
+"""Build libchns_amd.so in-tree for gfx950.
+
+    python ternary-chns_amd/build.py
+
+Compiles the host tables with g++ (-ffp-contract=off: the ETD2RK coefficients
+must round exactly like the reference), the kernels with hipcc for gfx950 and
+links one shared library into ``chns_amd/libchns_amd.so``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+OUT = os.path.join(HERE, "chns_amd", "libchns_amd.so")
+INC = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# packed FP32 instructions are disabled for every HIP translation unit
+NO_PK = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + os.path.basename(cmd[-1]))
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    objs = {
+        "tables": os.path.join(BUILD, "chns_tables.o"),
+        "api": os.path.join(BUILD, "chns_api.o"),
+        "device": os.path.join(BUILD, "chns_device.o"),
+    }
+    steps = [
+        ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INC,
+         "-c", os.path.join(CSRC, "chns_tables.cpp"), "-o", objs["tables"]],
+        [HIPCC, "-O2", "-fPIC", "-std=c++17", "-I" + INC, "-I/opt/rocm/include", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+         "-c", os.path.join(CSRC, "chns_api.cpp"), "-o", objs["api"]],
+        [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
+         "-c", os.path.join(CSRC, "chns_device.hip"), "-o", objs["device"]],
+    ]
+    with ThreadPoolExecutor(3) as ex:
+        list(ex.map(_run, steps))
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["api"],
+          objs["tables"]])
+    if verbose:
+        print("built", OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    build(verbose=True)

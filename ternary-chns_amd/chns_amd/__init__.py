@@ -1,0 +1,222 @@
+"""Python host side of the MI355X CHNS-2D hot path (ctypes over libchns_amd.so).
+
+The library is the product: every numeric operation below runs in the HIP
+kernels behind the C ABI declared in ``include/chns.h``.  This module only
+moves arrays across the boundary and mirrors the reference's entry points
+(``etd2rk``, ``truncation`` of the state as ``spectrum`` does, the
+``INIT_READ_DATA`` initialisation) for Python callers and tests.  If the
+shared library is missing the import fails loudly; there is no fallback.
+
+Array conventions are the reference's: complex half-spectra ``[Nx][Ny/2+1]``
+(``IDXC``, global.h:13) as numpy complex128, real fields ``[Nx][Ny]`` float64.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libchns_amd.so")
+
+MODEL_NS, MODEL_BINARY, MODEL_TERNARY = 0, 1, 2
+_MODELS = {"ns": MODEL_NS, "binary": MODEL_BINARY, "ternary": MODEL_TERNARY}
+
+CHNS_OK, CHNS_E_ARG, CHNS_E_HIP, CHNS_E_STATE = 0, -1, -2, -3
+
+EXPORTS = (
+    "chns_params_default", "chns_create", "chns_destroy", "chns_last_error", "chns_set_state",
+    "chns_get_state", "chns_set_real_state", "chns_step", "chns_sync", "chns_truncate_state",
+    "chns_etd_tables", "chns_dealias_extent",
+)
+
+
+class CParams(ctypes.Structure):
+    """Mirror of ``chns_params`` (include/chns.h)."""
+
+    _fields_ = [
+        ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("model", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("nu", ctypes.c_double), ("dt", ctypes.c_double),
+        ("sigma", ctypes.c_double), ("epsilon", ctypes.c_double), ("lambda_", ctypes.c_double),
+        ("sigma_12", ctypes.c_double), ("sigma_13", ctypes.c_double), ("sigma_23", ctypes.c_double),
+        ("Xlambda", ctypes.c_double),
+        ("smooth_filter", ctypes.c_int),
+        ("f_amp", ctypes.c_double), ("f_k", ctypes.c_int),
+    ]
+
+
+class ChnsError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libchns_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ChnsError(f"{path} is missing: run `python ternary-chns_amd/build.py` (no CPU fallback exists)")
+    lib = ctypes.CDLL(path)
+    P, D, I = ctypes.POINTER(CParams), ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    V = ctypes.c_void_p
+    lib.chns_params_default.argtypes = [P, I]
+    lib.chns_params_default.restype = None
+    lib.chns_create.argtypes = [ctypes.POINTER(V), P, I]
+    lib.chns_destroy.argtypes = [V]
+    lib.chns_destroy.restype = None
+    lib.chns_last_error.argtypes = [V]
+    lib.chns_last_error.restype = ctypes.c_char_p
+    lib.chns_set_state.argtypes = [V, D, D, D]
+    lib.chns_get_state.argtypes = [V, D, D, D]
+    lib.chns_set_real_state.argtypes = [V, D, D, D]
+    lib.chns_step.argtypes = [V, I]
+    lib.chns_sync.argtypes = [V]
+    lib.chns_truncate_state.argtypes = [V]
+    lib.chns_etd_tables.argtypes = [P, D, D, D, D, D, D]
+    lib.chns_dealias_extent.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    _lib = lib
+    return lib
+
+
+def params(model="ternary", **kw) -> CParams:
+    """chns_params with the reference defaults for ``model``, then overrides.
+
+    Keyword ``lam`` (or ``lambda_``) is the mobility ``lambda``; ``epsilon``
+    None or <= 0 selects 3*dx.
+    """
+    lib = load_library()
+    m = _MODELS[model] if isinstance(model, str) else int(model)
+    p = CParams()
+    lib.chns_params_default(ctypes.byref(p), m)
+    for k, v in kw.items():
+        if k == "lam":
+            k = "lambda_"
+        if k == "epsilon" and v is None:
+            v = -1.0
+        if k == "model":
+            v = _MODELS[v] if isinstance(v, str) else int(v)
+        setattr(p, k, v)
+    return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+def _nphase(model):
+    return {MODEL_NS: 0, MODEL_BINARY: 1, MODEL_TERNARY: 2}[model]
+
+
+class Context:
+    """A device-resident CHNS state (one ``chns_ctx``)."""
+
+    def __init__(self, p: CParams, device: int = 0):
+        self.lib = load_library()
+        self.p = p
+        self.Nx, self.Ny = p.Nx, p.Ny
+        self.Ny2 = p.Ny // 2 + 1
+        self.nph = _nphase(p.model)
+        h = ctypes.c_void_p()
+        rc = self.lib.chns_create(ctypes.byref(h), ctypes.byref(p), device)
+        if rc != CHNS_OK:
+            raise ChnsError(f"chns_create failed ({rc})")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != CHNS_OK:
+            msg = self.lib.chns_last_error(self.h).decode()
+            raise ChnsError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.chns_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- state ------------------------------------------------------------
+    def _cplx(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128)
+        if a.shape != (self.Nx, self.Ny2):
+            raise ValueError(f"expected shape {(self.Nx, self.Ny2)}, got {a.shape}")
+        return a.view(np.float64)
+
+    def set_state(self, cwz, cphi1=None, cphi2=None):
+        arrs = [self._cplx(cwz)]
+        arrs += [self._cplx(cphi1) if self.nph >= 1 else None, self._cplx(cphi2) if self.nph >= 2 else None]
+        self._check(self.lib.chns_set_state(self.h, *[_dp(a) for a in arrs]), "chns_set_state")
+
+    def get_state(self):
+        outs = [np.empty((self.Nx, self.Ny2), np.complex128) for _ in range(1 + self.nph)]
+        ptrs = [_dp(o.view(np.float64)) for o in outs] + [None] * (2 - self.nph)
+        self._check(self.lib.chns_get_state(self.h, *ptrs), "chns_get_state")
+        return outs
+
+    def set_real_state(self, wz, phi1=None, phi2=None):
+        def real(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != (self.Nx, self.Ny):
+                raise ValueError("real fields are [Nx][Ny]")
+            return a
+        arrs = [real(wz), real(phi1) if self.nph >= 1 else None, real(phi2) if self.nph >= 2 else None]
+        self._check(self.lib.chns_set_real_state(self.h, *[_dp(a) for a in arrs]), "chns_set_real_state")
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.chns_step(self.h, int(nsteps)), "chns_step")
+
+    def sync(self):
+        self._check(self.lib.chns_sync(self.h), "chns_sync")
+
+    def truncate_state(self):
+        self._check(self.lib.chns_truncate_state(self.h), "chns_truncate_state")
+
+
+# ---------------------------------------------------------------- reference-named mirrors
+def etd2rk(ctx: Context, nsteps: int = 1):
+    """etd2rk(cwz, cphi1, cphi2) (etd2rk.c:14-224) on the device-resident state."""
+    ctx.step(nsteps)
+
+
+def spectrum_truncate(ctx: Context):
+    """State side effect of spectrum() (spectrum.c:22-27)."""
+    ctx.truncate_state()
+
+
+def etd_tables(p: CParams):
+    """Host-built ETD2RK coefficient tables (IDXC order), no device needed."""
+    lib = load_library()
+    shape = (p.Nx, p.Ny // 2 + 1)
+    names = ("eta", "z1w", "z2w", "beta", "z1c", "z2c")
+    out = {k: np.zeros(shape) for k in names}
+    rc = lib.chns_etd_tables(ctypes.byref(p), *[_dp(out[k]) for k in names])
+    if rc != CHNS_OK:
+        raise ChnsError(f"chns_etd_tables failed ({rc})")
+    return out
+
+
+def dealias_extent(p: CParams):
+    """Largest kept |kx index| per ky column (-1: column fully truncated)."""
+    lib = load_library()
+    a = np.zeros(p.Ny // 2 + 1, dtype=np.int32)
+    rc = lib.chns_dealias_extent(ctypes.byref(p), a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc != CHNS_OK:
+        raise ChnsError(f"chns_dealias_extent failed ({rc})")
+    return a
+
+
+def two_pi():
+    return 2.0 * math.pi

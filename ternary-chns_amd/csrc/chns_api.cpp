@@ -1,0 +1,297 @@
+// C ABI of the CHNS-2D ETD2RK hot path: context, device buffers, state
+// transfer and the per-step launch sequence.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chns.h"
+#include "chns_host.hpp"
+#include "chns_launch.hpp"
+#include "chns_types.hpp"
+
+struct chns_ctx {
+  chns_params p{};
+  chns_host::Derived d{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int Jt = 0, ldJ = 0, ldF = 0, nph = 0;
+  chns::KParams kp{};
+  chns::Bufs b{};
+  std::vector<void*> allocs;
+  std::vector<int> amask;
+  bool have_state = false, hc_valid = false;
+  std::string err;
+};
+
+namespace {
+
+int fail(chns_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(chns_ctx* c, hipError_t e, const char* what) {
+  return fail(c, CHNS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define CHNS_HIP(ctx, call)                                          \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call);         \
+  } while (0)
+
+#define CHNS_LAUNCH(ctx, call)                                                        \
+  do {                                                                                \
+    int e_ = (call);                                                                  \
+    if (e_ != 0) return hip_fail((ctx), (hipError_t)e_, #call);                      \
+  } while (0)
+
+int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+template <class T> int dalloc(chns_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+  c->allocs.push_back(q);
+  e = hipMemset(q, 0, count * sizeof(T));
+  if (e != hipSuccess) return hip_fail(c, e, "hipMemset");
+  *p = (T*)q;
+  return 0;
+}
+
+// host IDXC [N][H] complex <-> device column-major [H][N]
+void to_cm(const double* src, std::vector<double>& dst, int N, int H) {
+  dst.resize((size_t)2 * N * H);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < H; ++j) {
+      dst[2 * ((size_t)j * N + i)] = src[2 * ((size_t)i * H + j)];
+      dst[2 * ((size_t)j * N + i) + 1] = src[2 * ((size_t)i * H + j) + 1];
+    }
+}
+void from_cm(const std::vector<double>& src, double* dst, int N, int H) {
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < H; ++j) {
+      dst[2 * ((size_t)i * H + j)] = src[2 * ((size_t)j * N + i)];
+      dst[2 * ((size_t)i * H + j) + 1] = src[2 * ((size_t)j * N + i) + 1];
+    }
+}
+
+void fill_kparams(chns_ctx* c) {
+  const chns_params& p = c->p;
+  const chns_host::Derived& d = c->d;
+  chns::KParams& k = c->kp;
+  k.N = d.N;
+  k.ldJ = c->ldJ;
+  k.ldF = c->ldF;
+  k.Jt = c->Jt;
+  k.f_k = p.f_amp != 0.0 ? p.f_k : 0;
+  k.f = d.factor_x;
+  k.scale = 1.0 / ((double)p.Nx * p.Ny);                  // fft.c:21
+  k.force_im = -0.5 * p.f_amp * (double)p.Nx * (double)p.Ny;
+  const double eps = d.eps;
+  k.g1 = d.g1; k.g2 = d.g2; k.g3 = d.g3;
+  if (p.model == CHNS_MODEL_TERNARY) {
+    k.g1i = 1.0 / d.g1; k.g2i = 1.0 / d.g2; k.g3i = 1.0 / d.g3;   // chem_potential.c:21-23
+    k.fac = 2.0 * d.del / eps;                                    // chem_potential.c:85-86
+    k.X6 = 6.0 * p.Xlambda;                                       // chem_potential.c:78-80
+    k.mc1 = 0.75 * d.g1 * eps;                                    // surface_tension.c:123
+    k.mc2 = 0.75 * d.g2 * eps;                                    // surface_tension.c:124
+    k.A11 = (1.0 + (d.g3 / d.g1)); k.A12 = (d.g3 / d.g2);         // surface_tension.c:144
+    k.A21 = (d.g3 / d.g1); k.A22 = (1.0 + (d.g3 / d.g2));         // surface_tension.c:195
+    k.lg1 = p.lambda / d.g1; k.lg2 = p.lambda / d.g2;             // nonlin.c:369-370
+  }
+  k.mcb = 1.5 * p.sigma * eps;                                    // surface_tension.c:141
+  k.s24 = 24.0 * (p.sigma / eps);                                 // surface_tension.c:142, nonlin.c:240
+  k.lam = p.lambda;                                               // nonlin.c:251
+}
+
+}  // namespace
+
+extern "C" const char* chns_last_error(const chns_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+extern "C" void chns_destroy(chns_ctx* c) {
+  if (!c) return;
+  for (void* q : c->allocs) (void)hipFree(q);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+extern "C" int chns_create(chns_ctx** out, const chns_params* p, int device) {
+  if (!out) return CHNS_E_ARG;
+  *out = nullptr;
+  if (const char* why = chns_host::check_params(p)) {
+    std::fprintf(stderr, "chns_create: %s\n", why);
+    return CHNS_E_ARG;
+  }
+  chns_ctx* c = new chns_ctx();
+  c->p = *p;
+  c->dev = device;
+  chns_host::derive(p, &c->d);
+  if (c->p.epsilon <= 0.0) c->p.epsilon = c->d.eps;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "chns_create: hipSetDevice(%d): %s\n", device, hipGetErrorString(e));
+    delete c;
+    return CHNS_E_HIP;
+  }
+  int rc = 0;
+  auto bail = [&](int r) { int code = r; std::fprintf(stderr, "chns_create: %s\n", c->err.c_str()); chns_destroy(c); return code; };
+  e = hipStreamCreate(&c->stream);
+  if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
+
+  const int N = c->d.N, H = c->d.H;
+  c->amask.assign(H, -1);
+  chns_host::dealias_extent(p, c->amask.data());
+  c->Jt = 0;
+  for (int j = 0; j < H; ++j)
+    if (c->amask[j] >= 0) c->Jt = j + 1;
+  if (c->Jt == 0) return bail(fail(c, CHNS_E_ARG, "dealias mask keeps no mode"));
+  c->ldJ = round_up(c->Jt, 4);
+  c->ldF = round_up(H, 4);
+  c->nph = p->model == CHNS_MODEL_TERNARY ? 2 : (p->model == CHNS_MODEL_BINARY ? 1 : 0);
+  fill_kparams(c);
+
+  chns::Bufs& b = c->b;
+  const size_t S = (size_t)N * H, RJ = (size_t)N * c->ldJ, RF = (size_t)N * c->ldF;
+  if ((rc = dalloc(c, &b.W, S)) || (rc = dalloc(c, &b.NW, S))) return bail(rc);
+  if (c->nph >= 1 && ((rc = dalloc(c, &b.C1, S)) || (rc = dalloc(c, &b.NC1, S)))) return bail(rc);
+  if (c->nph >= 2 && ((rc = dalloc(c, &b.C2, S)) || (rc = dalloc(c, &b.NC2, S)))) return bail(rc);
+  chns::cd** rj[] = {&b.Hc1, &b.Hc2, &b.Hu, &b.Hv, &b.Hwx, &b.Hwy, &b.Hmu1, &b.Hmu2, &b.Hdx1, &b.Hdx2};
+  for (auto q : rj)
+    if ((rc = dalloc(c, q, RJ))) return bail(rc);
+  chns::cd** rf[] = {&b.Mh1, &b.Mh2, &b.G1, &b.G2, &b.G3, &b.G4};
+  for (auto q : rf)
+    if ((rc = dalloc(c, q, RF))) return bail(rc);
+
+  std::vector<double> tw(2 * (size_t)N), t1(4 * (size_t)H * H), t2(2 * (size_t)H * H);
+  chns_host::build_twiddles(N, tw.data());
+  chns_host::build_coef_tables(&c->p, t1.data(), t2.data());
+  chns::cd* dtw;
+  double *dt1, *dt2;
+  int* dam;
+  if ((rc = dalloc(c, &dtw, N)) || (rc = dalloc(c, &dt1, t1.size())) || (rc = dalloc(c, &dt2, t2.size())) ||
+      (rc = dalloc(c, &dam, H)))
+    return bail(rc);
+  if ((e = hipMemcpy(dtw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dt1, t1.data(), t1.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dt2, t2.data(), t2.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dam, c->amask.data(), H * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+    return bail(hip_fail(c, e, "hipMemcpy(tables)"));
+  b.tw = dtw;
+  b.tab1 = (const chns::coef4*)dt1;
+  b.tab2 = (const chns::coef2*)dt2;
+  b.amask = dam;
+  *out = c;
+  return CHNS_OK;
+}
+
+extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi1, const double* cphi2) {
+  if (!c) return CHNS_E_ARG;
+  if (!cwz || (c->nph >= 1 && !cphi1) || (c->nph >= 2 && !cphi2)) return fail(c, CHNS_E_ARG, "null state array");
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  const int N = c->d.N, H = c->d.H;
+  std::vector<double> tmp;
+  const double* src[3] = {cwz, cphi1, cphi2};
+  chns::cd* dst[3] = {c->b.W, c->b.C1, c->b.C2};
+  for (int f = 0; f <= c->nph; ++f) {
+    to_cm(src[f], tmp, N, H);
+    CHNS_HIP(c, hipMemcpyAsync(dst[f], tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
+    CHNS_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  c->have_state = true;
+  c->hc_valid = false;
+  return CHNS_OK;
+}
+
+extern "C" int chns_get_state(chns_ctx* c, double* cwz, double* cphi1, double* cphi2) {
+  if (!c) return CHNS_E_ARG;
+  if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  const int N = c->d.N, H = c->d.H;
+  std::vector<double> tmp((size_t)2 * N * H);
+  double* dsts[3] = {cwz, cphi1, cphi2};
+  chns::cd* src[3] = {c->b.W, c->b.C1, c->b.C2};
+  for (int f = 0; f <= c->nph; ++f) {
+    if (!dsts[f]) continue;
+    CHNS_HIP(c, hipMemcpyAsync(tmp.data(), src[f], tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    CHNS_HIP(c, hipStreamSynchronize(c->stream));
+    from_cm(tmp, dsts[f], N, H);
+  }
+  return CHNS_OK;
+}
+
+extern "C" int chns_set_real_state(chns_ctx* c, const double* wz, const double* phi1, const double* phi2) {
+  if (!c) return CHNS_E_ARG;
+  if ((c->nph >= 1 && !phi1) || (c->nph >= 2 && !phi2)) return fail(c, CHNS_E_ARG, "null phase field");
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  const int N = c->d.N;
+  const size_t nb = (size_t)N * N * sizeof(double);
+  // The row-pass output buffers are free between steps: use Mh1/Mh2 as real
+  // staging ([N][N] doubles fit in [N][ldF] complex) and G1/G2 as the
+  // half-transformed rows.
+  double* ra = (double*)c->b.Mh1;
+  double* rb = (double*)c->b.Mh2;
+  auto dfft_trunc = [&](const double* a, const double* bb, chns::cd* oa, chns::cd* ob) -> int {
+    CHNS_HIP(c, hipMemcpyAsync(ra, a, nb, hipMemcpyHostToDevice, c->stream));
+    if (bb) CHNS_HIP(c, hipMemcpyAsync(rb, bb, nb, hipMemcpyHostToDevice, c->stream));
+    CHNS_LAUNCH(c, chns::launch_row_r2c(c->d.log2n, ra, bb ? rb : nullptr, c->b.G1, c->b.G2, c->b.tw, c->ldF,
+                                        c->stream));
+    CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G1, oa, c->b.tw, c->b.amask, c->ldF, c->stream));
+    if (bb) CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G2, ob, c->b.tw, c->b.amask, c->ldF,
+                                                c->stream));
+    CHNS_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  int rc;
+  if (wz) {
+    if ((rc = dfft_trunc(wz, nullptr, c->b.W, nullptr))) return rc;
+  } else {
+    CHNS_HIP(c, hipMemsetAsync(c->b.W, 0, (size_t)N * c->d.H * sizeof(chns::cd), c->stream));
+    CHNS_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (c->nph == 1 && (rc = dfft_trunc(phi1, nullptr, c->b.C1, nullptr))) return rc;
+  if (c->nph == 2 && (rc = dfft_trunc(phi1, phi2, c->b.C1, c->b.C2))) return rc;
+  c->have_state = true;
+  c->hc_valid = false;
+  return CHNS_OK;
+}
+
+extern "C" int chns_step(chns_ctx* c, int nsteps) {
+  if (!c) return CHNS_E_ARG;
+  if (nsteps < 0) return fail(c, CHNS_E_ARG, "nsteps < 0");
+  if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  const int L = c->d.log2n, M = c->p.model;
+  if (nsteps > 0 && c->nph > 0 && !c->hc_valid) CHNS_LAUNCH(c, chns::launch_colA(L, c->b, c->kp, c->nph, c->stream));
+  for (int n = 0; n < nsteps; ++n) {
+    for (int st = 1; st <= 2; ++st) {
+      if (c->nph > 0) CHNS_LAUNCH(c, chns::launch_rowB(L, M, c->b, c->kp, c->stream));
+      CHNS_LAUNCH(c, chns::launch_colC(L, M, c->b, c->kp, c->stream));
+      CHNS_LAUNCH(c, chns::launch_rowD(L, M, c->b, c->kp, c->stream));
+      CHNS_LAUNCH(c, chns::launch_colE(L, M, st, c->b, c->kp, 1 + c->nph, c->stream));
+    }
+  }
+  if (nsteps > 0) c->hc_valid = true;
+  return CHNS_OK;
+}
+
+extern "C" int chns_sync(chns_ctx* c) {
+  if (!c) return CHNS_E_ARG;
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
+  return CHNS_OK;
+}
+
+extern "C" int chns_truncate_state(chns_ctx* c) {
+  if (!c) return CHNS_E_ARG;
+  if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  if (c->nph >= 1) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C1, c->b.amask, c->d.N, c->stream));
+  if (c->nph >= 2) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C2, c->b.amask, c->d.N, c->stream));
+  return CHNS_OK;
+}

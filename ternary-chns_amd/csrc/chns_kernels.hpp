@@ -1,0 +1,535 @@
+// Fused pass kernels of one ETD2RK stage (CHNS-2D, FP64, gfx950).
+//
+// Layouts in HBM (N = Nx = Ny, Ny2 = N/2+1):
+//   spectral state / saved N_n : column-major  S[j*N + i]   (j = ky index, i = kx index)
+//   row-pass inputs  (H*)      : row-major     H[x*ldJ + j], j < Jt (dealiased columns)
+//   row-pass outputs (Mh*, G*) : row-major     G[x*ldF + j], j < Ny2
+// Column kernels (x direction) read the state contiguously and touch the
+// row-major arrays in chunks of C consecutive columns; row kernels (y
+// direction) are contiguous everywhere.
+//
+// Passes of one stage (reference functions they replace):
+//   A  colA : T(c_i) -> column iFFT -> Hc_i               (chem_potential.c:27-32, nonlin.c:282-284)
+//   B  rowB : row c2r -> c -> muF (ternary) / P(c) (binary) -> row r2c -> Mh
+//             (chem_potential.c:31-91, surface_tension.c:20-35, nonlin.c:217-230)
+//   C  colC : column FFT of Mh -> mu = T(M + a k^2 c) -> column iFFT; column
+//             iFFT of T(i ky psi), T(-i kx psi), T(i kx w), T(i ky w), T(i kx c_i)
+//             (velocity.c, nonlin.c:30-48, surface_tension.c:116-136)
+//   D  rowD : row c2r of all dealiased fields, products u.grad w, A_i grad c_i,
+//             u.grad c_i, row r2c; W1 = -F(u.grad w) - i ky F(Fx)
+//             (nonlin.c:56-92,266-348, surface_tension.c:138-220)
+//   E  colE : column FFT, N_w = W1 + i kx Fy, N_c = -adv - (lam/g) k^2 M,
+//             ETD2RK update with host-built coefficients (etd2rk.c:81-212),
+//             then the next stage's pass A for the dealiased columns.
+#pragma once
+#include "chns_fft.hpp"
+
+namespace chns {
+
+enum { MODEL_NS = 0, MODEL_BINARY = 1, MODEL_TERNARY = 2 };
+
+// exact (uncontracted) ksqr = kx*kx + ky*ky, as the host computes it
+__device__ __forceinline__ double ksq(double kx, double ky) {
+  return __dadd_rn(__dmul_rn(kx, kx), __dmul_rn(ky, ky));
+}
+
+template <int LOG2N> __device__ __forceinline__ int absidx(int i) {
+  constexpr int N = 1 << LOG2N;
+  return i <= N / 2 ? i : N - i;
+}
+template <int LOG2N> __device__ __forceinline__ double kxof(int i, double f) {
+  constexpr int N = 1 << LOG2N;
+  return f * (double)(i <= N / 2 ? i : i - N);   // functions.c:19-25
+}
+
+// columns per column-kernel workgroup
+template <int LOG2N> struct ColCfg {
+  static constexpr int N = 1 << LOG2N;
+  static constexpr int T = N / 8;
+  static constexpr int C = (LOG2N >= 12) ? 2 : 4;
+  static constexpr int NT = C * T;
+};
+// rows per row-kernel workgroup
+template <int LOG2N> struct RowCfg {
+  static constexpr int N = 1 << LOG2N;
+  static constexpr int T = N / 8;
+  static constexpr int R0 = (256 / T) > 0 ? (256 / T) : 1;
+  static constexpr int R = R0 < N ? R0 : N;
+  static constexpr int NT = R * T;
+};
+
+// ---------------------------------------------------------------- column I/O
+template <int LOG2N>
+__device__ __forceinline__ void col_load_rm(const cd* __restrict__ A, int ld, int j, bool ok, int t, cd* v) {
+  constexpr int T = (1 << LOG2N) / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = ok ? A[(size_t)(t + m * T) * ld + j] : mk(0.0, 0.0);
+}
+template <int LOG2N>
+__device__ __forceinline__ void col_store_rm(cd* __restrict__ A, int ld, int j, bool ok, int t, const cd* v) {
+  constexpr int T = (1 << LOG2N) / 8;
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) A[(size_t)(t + m * T) * ld + j] = v[m];
+  }
+}
+// masked load of a state column: modes with |i| > a are truncated (functions.c:75-78)
+template <int LOG2N>
+__device__ __forceinline__ void col_load_masked(const cd* __restrict__ S, int j, int a, int t, cd* v) {
+  constexpr int N = 1 << LOG2N;
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int i = t + m * T;
+    v[m] = (absidx<LOG2N>(i) <= a) ? S[(size_t)j * N + i] : mk(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------- row I/O
+// Assemble Z = X + iY (length N, Hermitian halves) from two dealiased
+// half-spectra rows so one complex c2r yields two real rows.  Only real parts
+// are used at ky = 0 and ky = N/2, as a c2r does.  YIK: Y is i*ky*Yrow.
+template <int LOG2N, bool YIK>
+__device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd* __restrict__ Y, int Jt,
+                                              double f, int t, cd* v) {
+  constexpr int N = 1 << LOG2N;
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int n = t + m * T;
+    const int k = n <= N / 2 ? n : N - n;
+    cd a = mk(0.0, 0.0), b = mk(0.0, 0.0);
+    if (k < Jt) {
+      a = X[k];
+      if (Y) b = Y[k];
+      if (YIK) b = ctimes_ik(b, f * (double)k);
+    }
+    if (n == 0 || n == N / 2) v[m] = mk(a.x, b.x);
+    else if (n < N / 2) v[m] = mk(a.x - b.y, a.y + b.x);
+    else v[m] = mk(a.x + b.y, b.x - a.y);
+  }
+}
+
+// Split the forward transform Z of a + ib into A = F(a), B = F(b) for
+// n = 0..N/2 and hand them to op(n, A, B).  Uses lds (N entries).
+template <int LOG2N, class Op>
+__device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
+  constexpr int N = 1 << LOG2N;
+  constexpr int T = N / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) lds[t + m * T] = v[m];
+  lds_sync();
+#pragma unroll
+  for (int m = 0; m < 5; ++m) {
+    const int n = t + m * T;
+    if (n <= N / 2) {
+      const cd zn = lds[n];
+      const cd zr = lds[(N - n) & (N - 1)];
+      op(n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
+    }
+  }
+}
+
+// ============================================================ pass A
+// grid: (ceil(Jt/C), nphase)
+template <int LOG2N>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colA(Bufs B, KParams P) {
+  using CF = ColCfg<LOG2N>;
+  constexpr int N = CF::N;
+  __shared__ cd lds_all[CF::C * N];
+  const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
+  const int j = blockIdx.x * CF::C + c;
+  const bool ok = j < P.Jt;
+  cd* lds = lds_all + c * N;
+  const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
+  cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
+  cd v[8];
+  col_load_masked<LOG2N>(S, j, ok ? B.amask[j] : -1, t, v);
+  fft<LOG2N, true>(v, lds, B.tw, t);
+  col_store_rm<LOG2N>(H, P.ldJ, j, ok, t, v);
+}
+
+// ============================================================ pass B
+// grid: N / R
+template <int LOG2N, int MODEL>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
+  using RC = RowCfg<LOG2N>;
+  constexpr int N = RC::N, T = RC::T;
+  __shared__ cd lds_all[RC::R * N];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int x = blockIdx.x * RC::R + r;
+  cd* lds = lds_all + r * N;
+  cd v[8];
+  const cd* h1 = B.Hc1 + (size_t)x * P.ldJ;
+  const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + (size_t)x * P.ldJ : nullptr;
+  row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
+  fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const double c1 = v[m].x * P.scale;
+    if constexpr (MODEL == MODEL_TERNARY) {
+      // chem_potential.c:40-86
+      const double c2 = v[m].y * P.scale;
+      const double c3 = (1.0 - c1 - c2);
+      const double c123 = c1 * c2 * c3;
+      const double fc1 = c1 * (1.0 - c1) * (1.0 - 2.0 * c1);
+      const double fc2 = c2 * (1.0 - c2) * (1.0 - 2.0 * c2);
+      const double fc3 = c3 * (1.0 - c3) * (1.0 - 2.0 * c3);
+      const double G1 = P.g1 * fc1 + P.X6 * c123 * c2 * c3;
+      const double G2 = P.g2 * fc2 + P.X6 * c123 * c1 * c3;
+      const double G3 = P.g3 * fc3 + P.X6 * c123 * c1 * c2;
+      const double m1 = (G1 - G2) * P.g2i + (G1 - G3) * P.g3i;
+      const double m2 = (G2 - G1) * P.g1i + (G2 - G3) * P.g3i;
+      v[m] = mk(m1 * P.fac, m2 * P.fac);
+    } else {
+      // P(c) = c(1-c)(1-2c)  (surface_tension.c:126-127, nonlin.c:225)
+      v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
+    }
+  }
+  fft<LOG2N, false>(v, lds, B.tw, t);
+  cd* o1 = B.Mh1 + (size_t)x * P.ldF;
+  cd* o2 = B.Mh2 + (size_t)x * P.ldF;
+  row_split<LOG2N>(v, lds, t, [&](int n, cd a, cd b) {
+    o1[n] = a;
+    if (MODEL == MODEL_TERNARY) o2[n] = b;
+  });
+}
+
+// ============================================================ pass C
+// grid: (ceil(Jt/C), groups): ternary 5 groups, binary 4, NS 2.
+template <int LOG2N, int MODEL>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
+  using CF = ColCfg<LOG2N>;
+  constexpr int N = CF::N, T = CF::T;
+  __shared__ cd lds_all[CF::C * N];
+  const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
+  const int j = blockIdx.x * CF::C + c;
+  const bool ok = j < P.Jt;
+  const int a = ok ? B.amask[j] : -1;
+  cd* lds = lds_all + c * N;
+  const double ky = P.f * (double)j;
+  int g = blockIdx.y;
+  if (MODEL == MODEL_NS) g += 2;                 // NS: only the velocity groups
+  if (MODEL == MODEL_BINARY && g >= 1) g += 1;   // binary: one mu group
+  cd v[8], s[8];
+  if (g <= 1) {
+    // mu_i = T(M_i + 0.75 g_i eps k^2 c_i)  (surface_tension.c:116-128)
+    // binary: mu = T(1.5 sigma eps k^2 c + 24 sigma/eps M)  (surface_tension.c:139-142,166)
+    col_load_rm<LOG2N>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    col_load_masked<LOG2N>(g == 0 ? B.C1 : B.C2, j, a, t, s);
+    const double mc = g == 0 ? P.mc1 : P.mc2;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * T;
+      const double ks = ksq(kxof<LOG2N>(i, P.f), ky);
+      if (absidx<LOG2N>(i) <= a) {
+        if (MODEL == MODEL_TERNARY) v[m] = cadd(v[m], cscale(s[m], mc * ks));
+        else v[m] = cadd(cscale(s[m], P.mcb * ks), cscale(v[m], P.s24));
+      } else {
+        v[m] = mk(0.0, 0.0);
+      }
+    }
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
+  } else if (g == 2) {
+    // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
+    col_load_masked<LOG2N>(B.W, j, a, t, s);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * T;
+      const double ks = ksq(kxof<LOG2N>(i, P.f), ky);
+      s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
+      v[m] = ctimes_ik(s[m], ky);
+    }
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(B.Hu, P.ldJ, j, ok, t, v);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(B.Hv, P.ldJ, j, ok, t, v);
+  } else if (g == 3) {
+    // grad w: i kx w, i ky w (nonlin.c:36-37)
+    col_load_masked<LOG2N>(B.W, j, a, t, s);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(B.Hwx, P.ldJ, j, ok, t, v);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], ky);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(B.Hwy, P.ldJ, j, ok, t, v);
+  } else {
+    // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
+    col_load_masked<LOG2N>(B.C1, j, a, t, s);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(B.Hdx1, P.ldJ, j, ok, t, v);
+    if constexpr (MODEL == MODEL_TERNARY) {
+      col_load_masked<LOG2N>(B.C2, j, a, t, s);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+      fft<LOG2N, true>(v, lds, B.tw, t);
+      col_store_rm<LOG2N>(B.Hdx2, P.ldJ, j, ok, t, v);
+    }
+  }
+}
+
+// ============================================================ pass D
+// grid: N / R
+template <int LOG2N, int MODEL>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
+  using RC = RowCfg<LOG2N>;
+  constexpr int N = RC::N, T = RC::T;
+  __shared__ cd lds_all[RC::R * N];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int x = blockIdx.x * RC::R + r;
+  cd* lds = lds_all + r * N;
+  const size_t ro = (size_t)x * P.ldJ;
+  const double sc = P.scale;
+  cd v[8];
+  double u[8], w[8], pw[8];
+  // u, v (nonlin.c:47-48)
+  row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+  fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
+  // grad w; P = u wx + v wy (nonlin.c:62)
+  row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
+  fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) pw[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
+  const cd* G1o = nullptr;
+  (void)G1o;
+  if constexpr (MODEL == MODEL_NS) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], 0.0);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o = B.G1 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd) { o[n] = mk(-A.x, -A.y); });  // nonlin.c:92
+  } else if constexpr (MODEL == MODEL_BINARY) {
+    double mu[8], fx[8], fy[8], adv[8];
+    row_load_pair<LOG2N, false>(B.Hmu1 + ro, nullptr, P.Jt, P.f, t, v);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) mu[m] = v[m].x * sc;
+    row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const double dx = v[m].x * sc, dy = v[m].y * sc;
+      fx[m] = dx * mu[m];                     // surface_tension.c:183-184
+      fy[m] = dy * mu[m];
+      adv[m] = u[m] * dx + w[m] * dy;         // nonlin.c:207
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], fx[m]);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o1 = B.G1 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bx) {
+      const cd iky = ctimes_ik(Bx, P.f * (double)n);
+      o1[n] = mk(-A.x - iky.x, -A.y - iky.y);
+    });
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], adv[m]);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o2 = B.G2 + (size_t)x * P.ldF;
+    cd* o3 = B.G3 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) { o2[n] = A; o3[n] = Bv; });
+  } else {
+    double A1[8], A2[8], fx[8], fy[8], ad1[8];
+    row_load_pair<LOG2N, false>(B.Hmu1 + ro, B.Hmu2 + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const double m1 = v[m].x * sc, m2 = v[m].y * sc;
+      A1[m] = (P.A11 * m1 + P.A12 * m2);     // surface_tension.c:144
+      A2[m] = (P.A21 * m1 + P.A22 * m2);     // surface_tension.c:195
+    }
+    row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const double dx = v[m].x * sc, dy = v[m].y * sc;
+      fx[m] = dx * A1[m];
+      fy[m] = dy * A1[m];
+      ad1[m] = dx * u[m] + dy * w[m];          // nonlin.c:292-293
+    }
+    row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const double dx = v[m].x * sc, dy = v[m].y * sc;
+      fx[m] += dx * A2[m];
+      fy[m] += dy * A2[m];
+      A2[m] = dx * u[m] + dy * w[m];           // adv2 (nonlin.c:334-335), reuses A2's registers
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], fx[m]);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o1 = B.G1 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bx) {
+      const cd iky = ctimes_ik(Bx, P.f * (double)n);
+      o1[n] = mk(-A.x - iky.x, -A.y - iky.y);
+    });
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o2 = B.G2 + (size_t)x * P.ldF;
+    cd* o3 = B.G3 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) { o2[n] = A; o3[n] = Bv; });
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(A2[m], 0.0);
+    fft<LOG2N, false>(v, lds, B.tw, t);
+    cd* o4 = B.G4 + (size_t)x * P.ldF;
+    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd) { o4[n] = A; });
+  }
+}
+
+// ============================================================ pass E
+// grid: (ceil(Ny2/C), fields): omega, then phase field(s).
+template <int LOG2N, int MODEL, int STAGE>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
+  using CF = ColCfg<LOG2N>;
+  constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
+  __shared__ cd lds_all[CF::C * N];
+  const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
+  const int j = blockIdx.x * CF::C + c;
+  const bool ok = j < H;
+  cd* lds = lds_all + c * N;
+  const double ky = P.f * (double)j;
+  const int g = blockIdx.y;
+  cd v[8], s[8];
+  const cd* Gm = g == 0 ? B.G1 : (g == 1 ? B.G3 : B.G4);
+  col_load_rm<LOG2N>(Gm, P.ldF, j, ok, t, v);
+  fft<LOG2N, false>(v, lds, B.tw, t);
+  if (g == 0) {
+    if constexpr (MODEL != MODEL_NS) {
+      // + i kx F(Fy): the curl of the surface-tension force (curl.c:26)
+      col_load_rm<LOG2N>(B.G2, P.ldF, j, ok, t, s);
+      fft<LOG2N, false>(s, lds, B.tw, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
+    }
+    if (P.f_k > 0 && j == P.f_k && t == 0) v[0].y += P.force_im;   // Kolmogorov forcing at (0, f_k)
+  } else {
+    // N_c = -F(u.grad c) - (lam/g) k^2 M  (nonlin.c:369-370); binary: - lam*(24 s/e k^2 M) (:240,251)
+    col_load_rm<LOG2N>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
+    fft<LOG2N, false>(s, lds, B.tw, t);
+    const double lg = g == 1 ? P.lg1 : P.lg2;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const double ks = ksq(kxof<LOG2N>(t + m * T, P.f), ky);
+      if (MODEL == MODEL_TERNARY) {
+        v[m] = csub(mk(-v[m].x, -v[m].y), cscale(s[m], lg * ks));
+      } else {
+        v[m] = csub(mk(-v[m].x, -v[m].y), cscale(cscale(s[m], P.s24 * ks), P.lam));
+      }
+    }
+  }
+  // ETD2RK update (etd2rk.c:99,113,129-130 stage 1; :174,188,205-206 stage 2)
+  cd* S = g == 0 ? B.W : (g == 1 ? B.C1 : B.C2);
+  cd* Nsv = g == 0 ? B.NW : (g == 1 ? B.NC1 : B.NC2);
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * T;
+      const size_t idx = (size_t)j * N + i;
+      const int ti = j * H + absidx<LOG2N>(i);
+      const cd u0 = S[idx];
+      if constexpr (STAGE == 1) {
+        const coef4 k4 = B.tab1[ti];
+        const double e = g == 0 ? k4.eta : k4.beta;
+        const double z = g == 0 ? k4.z1w : k4.z1c;
+        const cd a = cadd(cscale(u0, e), cscale(v[m], z));
+        S[idx] = a;
+        Nsv[idx] = v[m];
+        s[m] = a;
+      } else {
+        const coef2 k2 = B.tab2[ti];
+        const double z = g == 0 ? k2.z2w : k2.z2c;
+        const cd nn = Nsv[idx];
+        const cd a = cadd(u0, cscale(csub(v[m], nn), z));
+        S[idx] = a;
+        s[m] = a;
+      }
+    }
+  }
+  // next stage's pass A for the phase fields
+  if (g >= 1 && (int)(blockIdx.x * CF::C) < P.Jt) {
+    const bool okA = j < P.Jt;
+    const int a = okA ? B.amask[j] : -1;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
+    fft<LOG2N, true>(v, lds, B.tw, t);
+    col_store_rm<LOG2N>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
+  }
+}
+
+// ============================================================ dfft + truncation of real fields
+// Row r2c of packed real [N][N] fields a and b (b may be null), two rows per
+// complex transform; outputs row-major [N][ldF] half-spectra.  grid: N / R
+template <int LOG2N>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __restrict__ a,
+                                                               const double* __restrict__ b, cd* oa, cd* ob,
+                                                               const cd* __restrict__ tw, int ldF) {
+  using RC = RowCfg<LOG2N>;
+  constexpr int N = RC::N, T = RC::T;
+  __shared__ cd lds_all[RC::R * N];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const int x = blockIdx.x * RC::R + r;
+  cd* lds = lds_all + r * N;
+  cd v[8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const size_t o = (size_t)x * N + t + m * T;
+    v[m] = mk(a[o], b ? b[o] : 0.0);
+  }
+  fft<LOG2N, false>(v, lds, tw, t);
+  cd* pa = oa + (size_t)x * ldF;
+  cd* pb = ob ? ob + (size_t)x * ldF : nullptr;
+  row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) {
+    pa[n] = A;
+    if (pb) pb[n] = Bv;
+  });
+}
+
+// Column FFT of row-major half-transformed data into the column-major state,
+// with truncation() applied when TRUNC (init.c:256-262,291-294).
+// grid: (ceil(Ny2/C), 1)
+template <int LOG2N, bool TRUNC>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_fwd(const cd* __restrict__ in, cd* out,
+                                                              const cd* __restrict__ tw,
+                                                              const int* __restrict__ amask, int ldF) {
+  using CF = ColCfg<LOG2N>;
+  constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
+  __shared__ cd lds_all[CF::C * N];
+  const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
+  const int j = blockIdx.x * CF::C + c;
+  const bool ok = j < H;
+  cd* lds = lds_all + c * N;
+  cd v[8];
+  col_load_rm<LOG2N>(in, ldF, j, ok, t, v);
+  fft<LOG2N, false>(v, lds, tw, t);
+  if (ok) {
+    const int a = TRUNC ? amask[j] : N;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int i = t + m * T;
+      out[(size_t)j * N + i] = (absidx<LOG2N>(i) <= a) ? v[m] : mk(0.0, 0.0);
+    }
+  }
+}
+
+// In-place truncation of a column-major spectral field (spectrum.c:24-26).
+// grid: ceil(Ny2*N / 256)
+__global__ void k_truncate(cd* S, const int* __restrict__ amask, int N, int H) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * H) return;
+  const int j = (int)(e / N), i = (int)(e % N);
+  const int ai = i <= N / 2 ? i : N - i;
+  if (ai > amask[j]) S[e] = mk(0.0, 0.0);
+}
+
+}  // namespace chns

@@ -1,0 +1,22 @@
+// Launch wrappers between the host API (chns_api.cpp) and the kernels
+// (chns_device.hip).  Each returns the hipError_t of the launch as int.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+namespace chns {
+struct Bufs;
+struct KParams;
+struct cd;
+
+int launch_colA(int log2n, const Bufs& B, const KParams& P, int nphase, hipStream_t s);
+int launch_rowB(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nfields, hipStream_t s);
+int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int ldF,
+                   hipStream_t s);
+int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF,
+                   hipStream_t s);
+int launch_truncate(cd* S, const int* amask, int N, hipStream_t s);
+int col_tile(int log2n);   // columns per column-kernel workgroup
+}  // namespace chns

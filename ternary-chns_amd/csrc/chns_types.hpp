@@ -1,0 +1,44 @@
+// Plain-data types shared by the kernels (hipcc device pass) and the host API.
+#pragma once
+
+namespace chns {
+
+struct cd { double x, y; };   // complex double, same layout as double[2]
+
+struct alignas(16) coef4 { double eta, z1w, beta, z1c; };   // stage-1 ETD2RK coefficients
+struct alignas(16) coef2 { double z2w, z2c; };              // stage-2 ETD2RK coefficients
+
+struct KParams {
+  int N;         // grid size, Nx = Ny = N
+  int ldJ;       // leading dimension of row-pass inputs (>= Jt, multiple of 4)
+  int ldF;       // leading dimension of row-pass outputs (>= N/2+1, multiple of 4)
+  int Jt;        // number of columns holding kept (dealiased) modes
+  int f_k;       // forced ky column (0: no forcing)
+  double f;      // 2*pi/L
+  double scale;  // 1/(Nx*Ny)
+  double force_im;             // Im of the forcing half-spectrum at (0, f_k)
+  // ternary (chem_potential.c:21-23,74-86; surface_tension.c:123-124,144-145,195-196; nonlin.c:369-370)
+  double g1, g2, g3, g1i, g2i, g3i, fac, X6;
+  double mc1, mc2;             // 0.75*gamma_i*epsilon
+  double A11, A12, A21, A22;   // 1+g3/g1, g3/g2, g3/g1, 1+g3/g2
+  double lg1, lg2;             // lambda/gamma_i
+  // binary (surface_tension.c:141-142; nonlin.c:240,251)
+  double mcb;                  // 1.5*sigma*epsilon
+  double s24;                  // 24*(sigma/epsilon)
+  double lam;
+};
+
+struct Bufs {
+  cd *W, *C1, *C2;             // state (column-major [Ny2][N])
+  cd *NW, *NC1, *NC2;          // N_n saved by stage 1 (etd2rk.c:62-65)
+  cd *Hc1, *Hc2;               // row-major [N][ldJ]
+  cd *Hu, *Hv, *Hwx, *Hwy, *Hmu1, *Hmu2, *Hdx1, *Hdx2;
+  cd *Mh1, *Mh2;               // row-major [N][ldF]
+  cd *G1, *G2, *G3, *G4;       // W1, Fy, adv1, adv2 (row-major [N][ldF])
+  const cd* tw;                // twiddles exp(-2 pi i n/N)
+  const coef4* tab1;           // [(N/2+1)*(N/2+1)] indexed j*(N/2+1) + |i|
+  const coef2* tab2;
+  const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
+};
+
+}  // namespace chns
