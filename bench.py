@@ -1,0 +1,144 @@
+"""Benchmark: plain ETD2RK step throughput, ternary liquid lens 2048^2, FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line.  ``value`` is grid-point steps/s (N^2 * steps/s) with
+the state resident in HBM; the timed region is exactly K steps bracketed by
+device synchronisations of the library's stream.  ``roofline`` reports the
+dominant pass: algorithmic bytes per launch (the compulsory traffic of the
+five-pass design, coefficient tables excluded) over its mean launch duration
+from HIP events recorded on the library's stream during the timed region.
+``cpu_baseline`` times the numpy oracle (one core, pocketfft) on one step of
+the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ternary-chns_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(N=2048):
+    L = 2.0 * math.pi
+    return dict(Nx=N, Ny=N, lx=L, ly=L, nu=2e-3, dt=2.5e-4, lam=5e-4, epsilon=1.5 * L / N,
+                sigma_12=0.1, sigma_13=0.08, sigma_23=0.06, Xlambda=2.0)
+
+
+def pass_bytes(N, Jt, K):
+    """Algorithmic HBM bytes per launch of each pass (ternary)."""
+    F = N * (N // 2 + 1) * 16
+    Ft = N * Jt * 16
+    Fk = K * 16
+    return {
+        "colA": 2 * Fk + 2 * Ft,
+        "rowB": 2 * Ft + 2 * F,
+        "colC": 10 * Ft + 3 * Fk,
+        "rowD": 10 * Ft + 4 * F,
+        "colE1": 15 * F + 2 * Ft,
+        "colE2": 15 * F + 2 * Ft,
+    }, 42 * F + 48 * Ft + 6 * Fk
+
+
+def cpu_baseline(N, kw, seconds_cap=60.0):
+    import numpy as np  # noqa: F401
+
+    import oracle as O
+
+    op = O.Params(model="ternary", **kw)
+    c1, c2 = O.init_lens_real(op)
+    s = O.state_from_real(op, None, c1, c2)
+    tb = O.etd_tables(op)
+    t0 = time.perf_counter()
+    s = O.etd2rk(op, s, tb)
+    dt = time.perf_counter() - t0
+    steps = 1
+    return {"value": N * N * steps / dt, "unit": "grid-pt steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 ternary ETD2RK step at {N}^2 (numpy oracle, pocketfft, 1 thread), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--N", type=int, default=2048)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
+              "a single 2048^2 grid is not sharded across GPUs here", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import chns_amd as C
+    from chns_amd import ic
+
+    N = a.N
+    kw = workload(N)
+    p = C.params("ternary", **kw)
+    amask = C.dealias_extent(p)
+    Jt = int(np.max(np.where(amask >= 0)[0])) + 1
+    Kkept = int(np.sum(np.where(amask >= 0, 2 * amask + 1 - (amask >= N // 2), 0)))
+    c1, c2 = ic.lens(N, N, kw["lx"], kw["ly"], kw["epsilon"])
+    with C.Context(p, device=0) as ctx:
+        ctx.set_real_state(None, c1, c2)
+        ctx.step(a.warmup)
+        ctx.sync()
+        ctx.timing_read()
+        ctx.timing(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(a.steps)
+        ctx.sync()
+        t1 = time.perf_counter()
+        tm = ctx.timing_read()
+        ctx.timing(False)
+        state = ctx.get_state()
+    finite = all(bool(np.all(np.isfinite(f))) for f in state)
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    sps = a.steps / (t1 - t0)
+    pb, balg = pass_bytes(N, Jt, Kkept)
+    dom = max((k for k in tm if tm[k][1] > 0), key=lambda k: tm[k][0])
+    avg_ms = tm[dom][0] / tm[dom][1]
+    achieved = pb[dom] / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "grid-pt ETD2RK steps/s, ternary 2048^2 FP64",
+        "value": N * N * sps,
+        "unit": "grid-pt steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (liquid-lens IC, deterministic)",
+        "config": {"workload": f"ternary CHNS liquid lens {N}^2, ETD2RK, dt=2.5e-4, circular N/4 dealias",
+                   "N": N, "model": "ternary", "parallelism": "single GPU"},
+        "steps_per_s": sps,
+        "hbm_frac_step": balg * sps / (HBM_PEAK_GBS * 1e9),
+        "alg_bytes_per_step": balg,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "alg_bytes_per_launch": pb[dom], "avg_launch_ms": avg_ms},
+        "pass_ms_per_step": {k: tm[k][0] / a.steps for k in tm},
+        "finite": finite,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(N, kw)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

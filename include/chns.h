@@ -81,6 +81,22 @@ int chns_sync(chns_ctx *ctx);
  * spectrum() (spectrum.c:22-27) called at main.c:226,281-284. */
 int chns_truncate_state(chns_ctx *ctx);
 
+/* Per-pass kernel timing with HIP events recorded on the context's stream
+ * around every launch (measurement aid, off by default; no reference
+ * counterpart - the reference only times file output with MPI_Wtime,
+ * main.c:298-299,362-363).  Pass kinds: */
+#define CHNS_PASS_COLA 0   /* dealiased column iFFT of the phase fields */
+#define CHNS_PASS_ROWB 1   /* row c2r -> chemical potential -> row r2c */
+#define CHNS_PASS_COLC 2   /* column FFT/iFFTs of mu, velocity, gradients */
+#define CHNS_PASS_ROWD 3   /* row c2r -> nonlinear products -> row r2c */
+#define CHNS_PASS_COLE1 4  /* column FFT + ETD2RK stage 1 */
+#define CHNS_PASS_COLE2 5  /* column FFT + ETD2RK stage 2 */
+#define CHNS_NPASS 6
+int chns_timing_enable(chns_ctx *ctx, int enable);
+/* Synchronises, then returns the summed kernel milliseconds and launch
+ * counts per pass kind since the last read, and resets them. */
+int chns_timing_read(chns_ctx *ctx, double *total_ms, int *launches);
+
 /* ---- host-only helpers (no device needed) ---------------------------- */
 
 /* ETD2RK coefficients per mode, IDXC order [Nx][Ny2] each, in the exact

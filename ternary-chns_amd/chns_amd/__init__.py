@@ -29,8 +29,9 @@ CHNS_OK, CHNS_E_ARG, CHNS_E_HIP, CHNS_E_STATE = 0, -1, -2, -3
 EXPORTS = (
     "chns_params_default", "chns_create", "chns_destroy", "chns_last_error", "chns_set_state",
     "chns_get_state", "chns_set_real_state", "chns_step", "chns_sync", "chns_truncate_state",
-    "chns_etd_tables", "chns_dealias_extent",
+    "chns_etd_tables", "chns_dealias_extent", "chns_timing_enable", "chns_timing_read",
 )
+PASS_NAMES = ("colA", "rowB", "colC", "rowD", "colE1", "colE2")
 
 
 class CParams(ctypes.Structure):
@@ -79,6 +80,8 @@ def load_library(path: str = LIB_PATH):
     lib.chns_truncate_state.argtypes = [V]
     lib.chns_etd_tables.argtypes = [P, D, D, D, D, D, D]
     lib.chns_dealias_extent.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    lib.chns_timing_enable.argtypes = [V, I]
+    lib.chns_timing_read.argtypes = [V, D, ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -183,6 +186,19 @@ class Context:
 
     def truncate_state(self):
         self._check(self.lib.chns_truncate_state(self.h), "chns_truncate_state")
+
+    # -- measurement ------------------------------------------------------
+    def timing(self, enable: bool = True):
+        """Record HIP events around every pass launch on the context's stream."""
+        self._check(self.lib.chns_timing_enable(self.h, int(bool(enable))), "chns_timing_enable")
+
+    def timing_read(self):
+        """{pass: (total_ms, launches)} since the last read (synchronises)."""
+        ms = np.zeros(len(PASS_NAMES))
+        n = np.zeros(len(PASS_NAMES), dtype=np.int32)
+        self._check(self.lib.chns_timing_read(self.h, _dp(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
+                    "chns_timing_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(PASS_NAMES)}
 
 
 # ---------------------------------------------------------------- reference-named mirrors

@@ -24,6 +24,11 @@ struct chns_ctx {
   std::vector<int> amask;
   bool have_state = false, hc_valid = false;
   std::string err;
+  // event timing (chns_timing_enable)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kind;   // one entry per recorded pair
+  size_t ev_used = 0;
 };
 
 namespace {
@@ -108,6 +113,27 @@ void fill_kparams(chns_ctx* c) {
   k.lam = p.lambda;                                               // nonlin.c:251
 }
 
+// Launch through `fn`, bracketed by a pair of events when timing is on.
+template <class F> int timed(chns_ctx* c, int kind, F fn) {
+  if (!c->timing) return fn();
+  while (c->ev_pool.size() < 2 * (c->ev_used + 1)) {
+    hipEvent_t e;
+    hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) return (int)r;
+    c->ev_pool.push_back(e);
+  }
+  hipEvent_t a = c->ev_pool[2 * c->ev_used], b = c->ev_pool[2 * c->ev_used + 1];
+  hipError_t r = hipEventRecord(a, c->stream);
+  if (r != hipSuccess) return (int)r;
+  int rc = fn();
+  if (rc) return rc;
+  r = hipEventRecord(b, c->stream);
+  if (r != hipSuccess) return (int)r;
+  c->ev_kind.push_back(kind);
+  ++c->ev_used;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" const char* chns_last_error(const chns_ctx* ctx) {
@@ -116,6 +142,7 @@ extern "C" const char* chns_last_error(const chns_ctx* ctx) {
 
 extern "C" void chns_destroy(chns_ctx* c) {
   if (!c) return;
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* q : c->allocs) (void)hipFree(q);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -267,13 +294,18 @@ extern "C" int chns_step(chns_ctx* c, int nsteps) {
   if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
   CHNS_HIP(c, hipSetDevice(c->dev));
   const int L = c->d.log2n, M = c->p.model;
-  if (nsteps > 0 && c->nph > 0 && !c->hc_valid) CHNS_LAUNCH(c, chns::launch_colA(L, c->b, c->kp, c->nph, c->stream));
+  hipStream_t s = c->stream;
+  const chns::Bufs& B = c->b;
+  const chns::KParams& K = c->kp;
+  if (nsteps > 0 && c->nph > 0 && !c->hc_valid)
+    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLA, [&] { return chns::launch_colA(L, B, K, c->nph, s); }));
   for (int n = 0; n < nsteps; ++n) {
     for (int st = 1; st <= 2; ++st) {
-      if (c->nph > 0) CHNS_LAUNCH(c, chns::launch_rowB(L, M, c->b, c->kp, c->stream));
-      CHNS_LAUNCH(c, chns::launch_colC(L, M, c->b, c->kp, c->stream));
-      CHNS_LAUNCH(c, chns::launch_rowD(L, M, c->b, c->kp, c->stream));
-      CHNS_LAUNCH(c, chns::launch_colE(L, M, st, c->b, c->kp, 1 + c->nph, c->stream));
+      if (c->nph > 0) CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] { return chns::launch_rowB(L, M, B, K, s); }));
+      CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] { return chns::launch_colC(L, M, B, K, s); }));
+      CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] { return chns::launch_rowD(L, M, B, K, s); }));
+      CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2,
+                           [&] { return chns::launch_colE(L, M, st, B, K, 1 + c->nph, s); }));
     }
   }
   if (nsteps > 0) c->hc_valid = true;
@@ -293,5 +325,27 @@ extern "C" int chns_truncate_state(chns_ctx* c) {
   CHNS_HIP(c, hipSetDevice(c->dev));
   if (c->nph >= 1) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C1, c->b.amask, c->d.N, c->stream));
   if (c->nph >= 2) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C2, c->b.amask, c->d.N, c->stream));
+  return CHNS_OK;
+}
+
+extern "C" int chns_timing_enable(chns_ctx* c, int enable) {
+  if (!c) return CHNS_E_ARG;
+  c->timing = enable != 0;
+  return CHNS_OK;
+}
+
+extern "C" int chns_timing_read(chns_ctx* c, double* total_ms, int* launches) {
+  if (!c || !total_ms || !launches) return CHNS_E_ARG;
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < CHNS_NPASS; ++k) { total_ms[k] = 0.0; launches[k] = 0; }
+  for (size_t e = 0; e < c->ev_used; ++e) {
+    float ms = 0.f;
+    CHNS_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[2 * e], c->ev_pool[2 * e + 1]));
+    total_ms[c->ev_kind[e]] += ms;
+    launches[c->ev_kind[e]] += 1;
+  }
+  c->ev_used = 0;
+  c->ev_kind.clear();
   return CHNS_OK;
 }

@@ -69,6 +69,11 @@ template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw
   return INV ? mk(w.x, -w.y) : w;
 }
 
+// LDS index with one padding element after every 8: the radix-8 Stockham
+// writes at stride 8 (128 B) would otherwise hit the same banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
+template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + (1 << LOG2N) / 8; };
+
 // One Stockham stage of radix R on the cyclic registers v[8] of thread t.
 // Ns = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
@@ -86,8 +91,21 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
 #pragma unroll
     for (int m = 0; m < R; ++m) a[m] = v[u + m * P];
     if constexpr (NS > 1) {
+      // w^m for m < R from three table loads (w, w^2, w^4)
+      cd w[8];
+      w[1] = twid<INV>(tw, k * TWS);
+      if constexpr (R >= 4) {
+        w[2] = twid<INV>(tw, 2 * k * TWS);
+        w[3] = cmulc(w[1], w[2]);
+      }
+      if constexpr (R == 8) {
+        w[4] = twid<INV>(tw, 4 * k * TWS);
+        w[5] = cmulc(w[1], w[4]);
+        w[6] = cmulc(w[2], w[4]);
+        w[7] = cmulc(w[3], w[4]);
+      }
 #pragma unroll
-      for (int m = 1; m < R; ++m) a[m] = cmulc(a[m], twid<INV>(tw, m * k * TWS));
+      for (int m = 1; m < R; ++m) a[m] = cmulc(a[m], w[m]);
     }
     dftR<R, INV>(a);
     if constexpr (LAST) {
@@ -96,7 +114,7 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
     } else {
       const int base = (b / NS) * NS * R + k;
 #pragma unroll
-      for (int q = 0; q < R; ++q) lds[base + q * NS] = a[q];
+      for (int q = 0; q < R; ++q) lds[lp(base + q * NS)] = a[q];
     }
   }
 }
@@ -107,10 +125,10 @@ template <int LOG2N>
 __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
   constexpr int T = (1 << LOG2N) / 8;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = lds[t + m * T];
+  for (int m = 0; m < 8; ++m) v[m] = lds[lp(t + m * T)];
 }
 
-// Full transform.  v: cyclic input -> cyclic output.  lds: N entries private
+// Full transform.  v: cyclic input -> cyclic output.  lds: LdsSize entries private
 // to this transform.  Starts and ends with a workgroup barrier-free state:
 // the caller must synchronise before lds is reused by a different transform
 // (fft() itself begins with a barrier so back-to-back calls are safe).

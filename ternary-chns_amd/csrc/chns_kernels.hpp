@@ -111,20 +111,20 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
 }
 
 // Split the forward transform Z of a + ib into A = F(a), B = F(b) for
-// n = 0..N/2 and hand them to op(n, A, B).  Uses lds (N entries).
+// n = 0..N/2 and hand them to op(n, A, B).  Uses lds (LdsSize entries).
 template <int LOG2N, class Op>
 __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / 8;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) lds[t + m * T] = v[m];
+  for (int m = 0; m < 8; ++m) lds[lp(t + m * T)] = v[m];
   lds_sync();
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
     const int n = t + m * T;
     if (n <= N / 2) {
-      const cd zn = lds[n];
-      const cd zr = lds[(N - n) & (N - 1)];
+      const cd zn = lds[lp(n)];
+      const cd zr = lds[lp((N - n) & (N - 1))];
       op(n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
     }
   }
@@ -136,11 +136,11 @@ template <int LOG2N>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colA(Bufs B, KParams P) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N;
-  __shared__ cd lds_all[CF::C * N];
+  __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < P.Jt;
-  cd* lds = lds_all + c * N;
+  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
   const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
   cd v[8];
@@ -155,10 +155,10 @@ template <int LOG2N, int MODEL>
 __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * N];
+  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * N;
+  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
   cd v[8];
   const cd* h1 = B.Hc1 + (size_t)x * P.ldJ;
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + (size_t)x * P.ldJ : nullptr;
@@ -201,12 +201,12 @@ template <int LOG2N, int MODEL>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T;
-  __shared__ cd lds_all[CF::C * N];
+  __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < P.Jt;
   const int a = ok ? B.amask[j] : -1;
-  cd* lds = lds_all + c * N;
+  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
   const double ky = P.f * (double)j;
   int g = blockIdx.y;
   if (MODEL == MODEL_NS) g += 2;                 // NS: only the velocity groups
@@ -282,10 +282,10 @@ template <int LOG2N, int MODEL>
 __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * N];
+  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * N;
+  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
   const size_t ro = (size_t)x * P.ldJ;
   const double sc = P.scale;
   cd v[8];
@@ -393,11 +393,11 @@ template <int LOG2N, int MODEL, int STAGE>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
-  __shared__ cd lds_all[CF::C * N];
+  __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < H;
-  cd* lds = lds_all + c * N;
+  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
   const double ky = P.f * (double)j;
   const int g = blockIdx.y;
   cd v[8], s[8];
@@ -476,10 +476,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
                                                                const cd* __restrict__ tw, int ldF) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * N];
+  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * N;
+  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
   cd v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
@@ -504,11 +504,11 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_fwd(const cd* __restr
                                                               const int* __restrict__ amask, int ldF) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
-  __shared__ cd lds_all[CF::C * N];
+  __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < H;
-  cd* lds = lds_all + c * N;
+  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
   cd v[8];
   col_load_rm<LOG2N>(in, ldF, j, ok, t, v);
   fft<LOG2N, false>(v, lds, tw, t);

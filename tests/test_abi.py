@@ -1,0 +1,69 @@
+"""CPU checks of the C ABI library: it loads, exports every declared symbol,
+and its host-built tables match the oracle bit for bit (no device calls)."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import chns_amd as C
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "chns.h")).read()
+    return sorted(set(re.findall(r"\b(chns_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.load_library()
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+    assert set(declared_symbols()) == set(C.EXPORTS)
+
+
+CASES = [
+    ("ns", 64, 2 * math.pi), ("binary", 128, 2 * math.pi), ("ternary", 128, 4 * math.pi),
+    ("ternary", 2048, 2 * math.pi), ("binary", 512, 4 * math.pi),
+]
+
+
+@pytest.mark.parametrize("model,N,L", CASES)
+def test_etd_tables_bit_exact(model, N, L):
+    kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=2e-3, dt=2.5e-4, lam=5e-4, epsilon=1.5 * L / N, sigma=0.1,
+              sigma_12=0.1, sigma_13=0.08, sigma_23=0.06)
+    t = C.etd_tables(C.params(model, **kw))
+    o = O.etd_tables(O.Params(model=model, **kw))
+    for k in o:
+        assert np.array_equal(t[k].view(np.uint64), o[k].view(np.uint64)), k
+
+
+@pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
+def test_etd_tables_reference_defaults(model):
+    t = C.etd_tables(C.params(model))
+    o = O.etd_tables(O.Params(model=model))
+    for k in o:
+        assert np.array_equal(t[k].view(np.uint64), o[k].view(np.uint64)), k
+
+
+@pytest.mark.parametrize("N,L", [(64, 2 * math.pi), (128, 4 * math.pi), (2048, 2 * math.pi), (256, 3.0)])
+def test_dealias_extent_matches_mask(N, L):
+    a = C.dealias_extent(C.params("ternary", Nx=N, Ny=N, lx=L, ly=L))
+    m = O.dealias_mask(O.Params(Nx=N, Ny=N, lx=L, ly=L))
+    for j in range(N // 2 + 1):
+        kept = np.where(m[:, j])[0]
+        expect = -1 if kept.size == 0 else int(np.minimum(kept, N - kept).max())
+        assert a[j] == expect
+
+
+def test_invalid_params_rejected():
+    lib = C.load_library()
+    p = C.params("ternary", Nx=100, Ny=100)
+    out = np.zeros(100 * 51)
+    assert lib.chns_etd_tables(C.ctypes.byref(p), *[C._dp(out)] * 6) == C.CHNS_E_ARG
+    p = C.params("ternary", Nx=64, Ny=32)
+    assert lib.chns_dealias_extent(C.ctypes.byref(p), np.zeros(17, np.int32).ctypes.data_as(
+        C.ctypes.POINTER(C.ctypes.c_int))) == C.CHNS_E_ARG

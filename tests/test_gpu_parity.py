@@ -1,0 +1,164 @@
+"""GPU parity of the fused ETD2RK step against the oracle (through the C ABI).
+
+Bar (north star): relative L2 of omega, c1, c2 <= 1e-10 after 100 steps,
+mass (the k = 0 mode of each phase field) conserved to 1e-13.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import chns_amd as C
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_TRAJ = 1e-10     # rel-L2 after 100 steps (FP64)
+TOL_MASS = 1e-13     # relative drift of the k=0 mode
+
+
+def case(model, N, L=2 * math.pi, **over):
+    kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=5e-4, lam=1e-3, epsilon=1.5 * L / N,
+              sigma=0.1, sigma_12=0.1, sigma_13=0.08, sigma_23=0.06)
+    kw.update(over)
+    op = O.Params(model=model, **kw)
+    wz = O.random_vorticity(op, 1)
+    if model == "ternary":
+        c1, c2 = O.init_noise_real(op, 2, 1.0 / 3.0, 0.05)
+        reals = (wz, c1, c2)
+    elif model == "binary":
+        (c1,) = O.init_noise_real(op, 1, 0.5, 0.1)
+        reals = (wz, c1)
+    else:
+        reals = (wz,)
+    return op, C.params(model, **kw), reals
+
+
+def rel(a, b):
+    return O.rel_l2(a, b)
+
+
+@pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_init_dfft_truncation(model, N):
+    op, cp, reals = case(model, N)
+    s0 = O.state_from_real(op, *reals)
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(*reals)
+        g = ctx.get_state()
+    for a, b in zip(g, s0.fields()):
+        assert rel(a, b) < 1e-14
+
+
+@pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
+@pytest.mark.parametrize("N", [16, 32, 64, 128])
+def test_one_step_increment(model, N):
+    """The change of the state over one step matches to 1e-10 (tests N itself)."""
+    op, cp, reals = case(model, N)
+    s0 = O.state_from_real(op, *reals)
+    s1 = O.etd2rk(op, s0)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.step(1)
+        g = ctx.get_state()
+    for a, b, a0 in zip(g, s1.fields(), s0.fields()):
+        assert rel(a - a0, b - a0) < 1e-10
+
+
+@pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
+@pytest.mark.parametrize("N", [32, 64, 128])
+def test_100_steps(model, N):
+    op, cp, reals = case(model, N)
+    s0 = O.state_from_real(op, *reals)
+    ref = O.run(op, s0, 100)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.step(100)
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert rel(a, b) <= TOL_TRAJ
+    for a, b in zip(g[1:], s0.fields()[1:]):
+        assert abs(a[0, 0] - b[0, 0]) <= TOL_MASS * abs(b[0, 0])
+
+
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+def test_spectrum_truncation_cadence(model):
+    """spectrum() truncates the phase fields every navg/2 steps (main.c:281-284)."""
+    op, cp, reals = case(model, 64)
+    s0 = O.state_from_real(op, *reals)
+    ref = O.run(op, s0, 40, spectrum_every=10)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        for _ in range(4):
+            ctx.step(10)
+            ctx.truncate_state()
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert rel(a, b) <= TOL_TRAJ
+
+
+def test_reference_defaults_ternary():
+    """Reference defaults (L = 4 pi, eps = 3 dx, gamma1 < 0) at N = 64, glibc-rand IC."""
+    op = O.Params(model="ternary", Nx=64, Ny=64)
+    cp = C.params("ternary", Nx=64, Ny=64)
+    c1, c2 = O.init_ternary_noise_real(op, nranks=1)
+    s0 = O.state_from_real(op, None, c1, c2)
+    ref = O.run(op, s0, 100)
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(None, c1, c2)
+        ctx.step(100)
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert rel(a, b) <= TOL_TRAJ
+
+
+def test_kolmogorov_steady_state():
+    """NS + forcing f sin(k_f y): omega -> f/(nu k_f^2) sin(k_f y), exact under ETD."""
+    N, kf, F0, nu, dt = 64, 4, 0.1, 5e-3, 0.5
+    op, cp, _ = case("ns", N, nu=nu, dt=dt, f_amp=F0, f_k=kf)
+    zero = np.zeros((N, N // 2 + 1), np.complex128)
+    with C.Context(cp) as ctx:
+        ctx.set_state(zero)
+        ctx.step(4000)
+        (w,) = ctx.get_state()
+    t = 4000 * dt
+    amp = F0 / (nu * kf * kf) * (1.0 - math.exp(-nu * kf * kf * t))
+    expect = np.zeros_like(zero)
+    expect[0, kf] = -0.5j * amp * N * N
+    assert rel(w, expect) < 1e-12
+
+
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+@pytest.mark.parametrize("N", [256, 512])
+def test_larger_grids_10_steps(model, N):
+    op, cp, reals = case(model, N, dt=5e-5)
+    s0 = O.state_from_real(op, *reals)
+    ref = O.run(op, s0, 10)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.step(10)
+        g = ctx.get_state()
+    for a, b, a0 in zip(g, ref.fields(), s0.fields()):
+        assert rel(a, b) <= TOL_TRAJ
+        assert rel(a - a0, b - a0) < 1e-11
+
+
+@pytest.mark.parametrize("N", [1024, 2048])
+def test_full_size_properties(N):
+    """At the benchmark sizes: mass conservation, finiteness, bitwise repeatability."""
+    op, cp, _ = case("ternary", N, nu=2e-3, lam=5e-4, dt=1e-6)
+    c1, c2 = O.init_lens_real(op)
+    outs = []
+    for _ in range(2):
+        with C.Context(cp) as ctx:
+            ctx.set_real_state(None, c1, c2)
+            m0 = [f[0, 0] for f in ctx.get_state()[1:]]
+            ctx.step(20)
+            g = ctx.get_state()
+        outs.append(g)
+        for f in g:
+            assert np.all(np.isfinite(f))
+        for f, m in zip(g[1:], m0):
+            assert abs(f[0, 0] - m) <= TOL_MASS * abs(m)
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))

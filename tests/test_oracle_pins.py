@@ -1,0 +1,309 @@
+"""Pins of the oracle restatement, independent of how the oracle is written.
+
+Each check compares an oracle function with something the mathematics fixes:
+a direct O(N^4) DFT, integer lattice counts, sympy-derived real-space closed
+forms for band-limited fields (which the pseudo-spectral method must reproduce
+to round-off because nothing aliases or is truncated), exact rational
+arithmetic, Parseval, analytic ODE solutions, and published constants.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import sympy as sp
+
+import oracle as O
+
+TWO_PI = 2.0 * math.pi
+
+
+def grid(p):
+    x = (np.arange(p.Nx) * p.dx)[:, None] * np.ones((1, p.Ny))
+    y = (np.arange(p.Ny) * p.dy)[None, :] * np.ones((p.Nx, 1))
+    return x, y
+
+
+# ---------------------------------------------------------------- transforms
+def test_dfft_matches_direct_dft():
+    p = O.Params(Nx=8, Ny=8)
+    u = np.random.default_rng(0).standard_normal((8, 8))
+    i = np.arange(8)
+    E = np.exp(-2j * np.pi * np.outer(i, i) / 8)
+    full = E @ u @ E.T
+    assert np.allclose(O.dfft(u), full[:, :5], atol=1e-12, rtol=0)
+    assert np.allclose(O.ifft(p, O.dfft(u)), u, atol=1e-14)
+
+
+def test_single_mode_and_parseval():
+    p = O.Params(Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    x, y = grid(p)
+    U = O.dfft(np.cos(3 * x + 5 * y))
+    expect = np.zeros_like(U)
+    expect[3, 5] = 0.5 * 32 * 32
+    assert np.allclose(U, expect, atol=1e-10)
+    u = np.random.default_rng(1).standard_normal((32, 32))
+    Uh = O.dfft(u)
+    w = np.full(17, 2.0)
+    w[0] = w[16] = 1.0
+    assert math.isclose(np.sum(u * u), np.sum(w * np.abs(Uh) ** 2) / 32 ** 2, rel_tol=1e-13)
+
+
+# ---------------------------------------------------------------- kxky / truncation
+def test_kxky_nyquist_positive():
+    p = O.Params(Nx=16, Ny=16, lx=4 * math.pi, ly=4 * math.pi)
+    kx, ky = O.kxky(p)
+    assert kx[8] == 4.0 and kx[9] == -3.5 and ky[-1] == 4.0
+
+
+@pytest.mark.parametrize("N,K", [(128, 1637), (2048, 412249)])
+def test_truncation_keep_count_is_lattice_count(N, K):
+    p = O.Params(Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI)
+    R2 = (N // 4) ** 2
+    count = 0
+    for i in range(-N // 2 + 1, N // 2 + 1):
+        jmax = math.isqrt(max(R2 - i * i, -1)) if R2 - i * i >= 0 else -1
+        count += max(0, min(jmax, N // 2) + 1)
+    assert count == K
+    assert int(O.dealias_mask(p).sum()) == K
+
+
+# ---------------------------------------------------------------- velocity / curl
+def test_curl_of_velocity_is_vorticity_and_divergence_free():
+    p = O.Params(Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    w = O.truncation(p, O.dfft(O.random_vorticity(p, 3)))
+    w[0, 0] = 0.0
+    ux, uy = O.get_velocity(p, w)
+    assert O.rel_l2(O.curl(p, ux, uy), w) < 1e-15
+    assert np.max(np.abs(O.divergence(p, ux, uy))) < 1e-15 * np.max(np.abs(w))
+
+
+def test_single_mode_velocity():
+    p = O.Params(Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    x, y = grid(p)
+    ux, uy = O.get_velocity(p, O.dfft(2.0 * np.sin(x) + 3.0 * np.cos(2 * y)))
+    # psi = 2 sin x + (3/4) cos 2y ; u = dpsi/dy, v = -dpsi/dx
+    assert np.allclose(O.ifft(p, ux), -1.5 * np.sin(2 * y), atol=1e-14)
+    assert np.allclose(O.ifft(p, uy), -2.0 * np.cos(x), atol=1e-14)
+
+
+# ---------------------------------------------------------------- chemical potential
+X1, X2 = sp.symbols("c1 c2")
+
+
+def sym_muF(g1, g2, g3, dl, Xl, eps):
+    """muF from the ternary free energy, differentiated by sympy.
+
+    F0 = sum_i (g_i/2) c_i^2 (1-c_i)^2 + 3 Xl (c1 c2 c3)^2 with c1, c2, c3
+    independent; G_i = dF0/dc_i; muF1 = [(G1-G2)/g2 + (G1-G3)/g3] 2 del/eps.
+    """
+    a, b, c = sp.symbols("a b c")
+    F0 = (g1 / 2) * a**2 * (1 - a)**2 + (g2 / 2) * b**2 * (1 - b)**2 + (g3 / 2) * c**2 * (1 - c)**2 \
+        + 3 * Xl * (a * b * c)**2
+    G = [sp.diff(F0, v) for v in (a, b, c)]
+    sub = {a: X1, b: X2, c: 1 - X1 - X2}
+    G = [g.subs(sub) for g in G]
+    m1 = ((G[0] - G[1]) / g2 + (G[0] - G[2]) / g3) * 2 * dl / eps
+    m2 = ((G[1] - G[0]) / g1 + (G[1] - G[2]) / g3) * 2 * dl / eps
+    return sp.expand(m1), sp.expand(m2)
+
+
+def test_potential_pointwise_exact_rational():
+    p = O.Params(Nx=16, Ny=16, lx=TWO_PI, ly=TWO_PI, sigma_12=0.1, sigma_13=0.08, sigma_23=0.06,
+                 epsilon=0.0625)
+    g1, g2, g3, dl = p.gammas
+    m1s, m2s = sym_muF(*[sp.Rational(Fraction(v)) for v in (g1, g2, g3, dl, p.Xlambda, p.epsilon)])
+    rng = np.random.default_rng(5)
+    c1 = rng.integers(-256, 1280, size=50) / 1024.0
+    c2 = rng.integers(-256, 1280, size=50) / 1024.0
+    o1, o2 = O.potential_pointwise(p, c1, c2)
+    for k in range(50):
+        s = {X1: sp.Rational(Fraction(c1[k])), X2: sp.Rational(Fraction(c2[k]))}
+        e1, e2 = float(m1s.subs(s)), float(m2s.subs(s))
+        scale = max(1.0, abs(e1), abs(e2))
+        assert abs(o1[k] - e1) <= 1e-13 * scale
+        assert abs(o2[k] - e2) <= 1e-13 * scale
+
+
+def test_potential_zeros():
+    p = O.Params(Nx=16, Ny=16, sigma_12=0.1, sigma_13=0.1, sigma_23=0.1)
+    one3 = np.array([1.0 / 3.0])
+    m1, m2 = O.potential_pointwise(p, one3, one3)
+    assert abs(m1[0]) < 1e-15 and abs(m2[0]) < 1e-15
+    for a, b in ((1.0, 0.0), (0.0, 1.0), (0.0, 0.0)):
+        m1, m2 = O.potential_pointwise(p, np.array([a]), np.array([b]))
+        assert m1[0] == 0.0 and m2[0] == 0.0
+
+
+# ---------------------------------------------------------------- surface tension and nonlin (closed forms)
+xs, ys = sp.symbols("x y")
+
+
+def band_limited_case(model):
+    """Fields with Fourier modes <= 2 at N = 64: no product aliases, nothing truncated."""
+    p = O.Params(model=model, Nx=64, Ny=64, lx=TWO_PI, ly=TWO_PI, sigma=0.1, epsilon=0.2, lam=1e-3,
+                 sigma_12=0.1, sigma_13=0.08, sigma_23=0.06)
+    # mixed |k| so that grad(mu) is not parallel to grad(c); muF (degree 5) stays
+    # inside the dealias disk (|k| <= 10*sqrt(2) < 16) and products resolve on 64^2
+    c1 = sp.Rational(1, 3) + sp.Rational(1, 10) * sp.cos(xs) + sp.Rational(1, 20) * sp.sin(2 * ys)
+    c2 = sp.Rational(1, 3) - sp.Rational(1, 20) * sp.cos(ys) + sp.Rational(1, 25) * sp.sin(2 * xs)
+    if model == "binary":
+        c1 = sp.Rational(1, 2) + sp.Rational(1, 10) * sp.cos(xs) + sp.Rational(1, 20) * sp.sin(2 * ys)
+    w = sp.Rational(3, 10) * sp.sin(xs + ys) + sp.Rational(1, 5) * sp.cos(2 * ys)
+    return p, c1, c2, w
+
+
+def lap(f):
+    return sp.diff(f, xs, 2) + sp.diff(f, ys, 2)
+
+
+def velocity_of(w):
+    # psi with -lap(psi) = w for the modes used: sin(x+y) (k^2 = 2), cos(2y) (k^2 = 4)
+    psi = sp.Rational(3, 20) * sp.sin(xs + ys) + sp.Rational(1, 20) * sp.cos(2 * ys)
+    assert sp.simplify(-lap(psi) - w) == 0
+    return sp.diff(psi, ys), -sp.diff(psi, xs)
+
+
+def on_grid(expr, p):
+    f = sp.lambdify((xs, ys), expr, "numpy")
+    x, y = grid(p)
+    return np.broadcast_to(f(x, y), x.shape).astype(float)
+
+
+def analytic_rhs(model, p, c1, c2, w):
+    u, v = velocity_of(w)
+    eps = sp.Rational(Fraction(p.epsilon))
+    nw = -(u * sp.diff(w, xs) + v * sp.diff(w, ys))
+    if model == "ternary":
+        g1, g2, g3, dl = [sp.Rational(Fraction(q)) for q in p.gammas]
+        m1, m2 = sym_muF(g1, g2, g3, dl, sp.Rational(Fraction(p.Xlambda)), eps)
+        mf1 = m1.subs({X1: c1, X2: c2})
+        mf2 = m2.subs({X1: c1, X2: c2})
+        mu1 = mf1 - sp.Rational(3, 4) * g1 * eps * lap(c1)
+        mu2 = mf2 - sp.Rational(3, 4) * g2 * eps * lap(c2)
+        A1 = (1 + g3 / g1) * mu1 + (g3 / g2) * mu2
+        A2 = (g3 / g1) * mu1 + (1 + g3 / g2) * mu2
+        Fx = A1 * sp.diff(c1, xs) + A2 * sp.diff(c2, xs)
+        Fy = A1 * sp.diff(c1, ys) + A2 * sp.diff(c2, ys)
+        st = sp.diff(Fy, xs) - sp.diff(Fx, ys)
+        lam = sp.Rational(Fraction(p.lam))
+        n1 = -(u * sp.diff(c1, xs) + v * sp.diff(c1, ys)) + (lam / g1) * lap(mf1)
+        n2 = -(u * sp.diff(c2, xs) + v * sp.diff(c2, ys)) + (lam / g2) * lap(mf2)
+        return st, nw + st, n1, n2
+    sig = sp.Rational(Fraction(p.sigma))
+    P = c1 * (1 - c1) * (1 - 2 * c1)
+    mu = sp.Rational(3, 2) * sig * eps * (-lap(c1)) + 24 * (sig / eps) * P
+    st = sp.diff(mu * sp.diff(c1, ys), xs) - sp.diff(mu * sp.diff(c1, xs), ys)
+    lam = sp.Rational(Fraction(p.lam))
+    n1 = -(u * sp.diff(c1, xs) + v * sp.diff(c1, ys)) + lam * 24 * (sig / eps) * lap(P)
+    return st, nw + st, n1, None
+
+
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+def test_surface_tension_and_nonlin_closed_form(model):
+    p, c1, c2, w = band_limited_case(model)
+    st, nw, n1, n2 = analytic_rhs(model, p, c1, c2, w)
+    C1 = O.dfft(on_grid(c1, p))
+    C2 = O.dfft(on_grid(c2, p))
+    W = O.dfft(on_grid(w, p))
+    if model == "ternary":
+        M1, M2 = O.potential(p, C1, C2)
+        got_st = O.surface_tension(p, C1, C2, M1, M2)
+        got = O.nonlin(p, W, C1, C2, M1, M2)
+    else:
+        got_st = O.surface_tension(p, C1)
+        got = O.nonlin(p, W, C1)
+    ref_st = on_grid(st, p)
+    assert np.linalg.norm(ref_st) > 1e-3
+    assert O.rel_l2(O.ifft(p, got_st), ref_st) < 1e-10   # the source is a small difference of O(1) terms
+    assert O.rel_l2(O.ifft(p, got[0]), on_grid(nw, p)) < 1e-12
+    assert O.rel_l2(O.ifft(p, got[1]), on_grid(n1, p)) < 1e-12
+    if n2 is not None:
+        assert O.rel_l2(O.ifft(p, got[2]), on_grid(n2, p)) < 1e-12
+
+
+def test_surface_tension_vanishes_for_one_dimensional_fields():
+    p = O.Params(model="ternary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, sigma_12=0.1, sigma_13=0.08,
+                 sigma_23=0.06)
+    x, y = grid(p)
+    C1 = O.truncation(p, O.dfft(0.3 + 0.2 * np.tanh(3 * np.sin(y))))
+    C2 = O.truncation(p, O.dfft(0.3 - 0.1 * np.cos(2 * y) + 0.05 * np.sin(5 * y)))
+    M1, M2 = O.potential(p, C1, C2)
+    assert np.max(np.abs(O.surface_tension(p, C1, C2, M1, M2))) == 0.0
+
+
+def test_taylor_green_has_no_advection():
+    p = O.Params(model="ns", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    x, y = grid(p)
+    N = O.nonlin(p, O.dfft(2.0 * np.sin(x) * np.sin(y)))
+    assert np.max(np.abs(N)) < 1e-10
+
+
+# ---------------------------------------------------------------- ETD2RK
+def test_etd_coefficients_match_closed_form_where_well_conditioned():
+    p = O.Params(model="ternary", Nx=64, Ny=64, lx=TWO_PI, ly=TWO_PI, nu=0.3, dt=0.01, lam=0.5, epsilon=0.5)
+    tb = O.etd_tables(p)
+    ks = O.ksqr_grid(p)
+    c = -p.nu * ks
+    good = np.abs(c * p.dt) > 1e-2
+    ex = np.expm1(c[good] * p.dt)
+    assert np.allclose(tb["eta"][good], np.exp(c[good] * p.dt), rtol=1e-15, atol=0)
+    assert np.allclose(tb["z1w"][good], ex / c[good], rtol=1e-13, atol=0)
+    phi2 = (ex - c[good] * p.dt) / (p.dt * c[good] ** 2)
+    assert np.allclose(tb["z2w"][good], phi2, rtol=1e-9, atol=0)
+    assert tb["z1w"][0, 0] == p.dt and tb["z2w"][0, 0] == 0.5 * p.dt
+
+
+def test_stokes_decay_single_mode():
+    p = O.Params(model="ns", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, nu=0.05, dt=0.01)
+    x, y = grid(p)
+    s = O.State(O.dfft(np.cos(3 * x + 4 * y)))
+    s = O.run(p, s, 50)
+    expect = O.dfft(np.cos(3 * x + 4 * y)) * math.exp(-p.nu * 25 * 50 * p.dt)
+    assert O.rel_l2(s.cwz, expect) < 1e-13
+
+
+def test_forced_kolmogorov_steady_state():
+    p = O.Params(model="ns", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, nu=0.05, dt=0.5, f_amp=0.1, f_k=4)
+    s = O.run(p, O.State(np.zeros((32, 17), np.complex128)), 400)
+    x, y = grid(p)
+    amp = 0.1 / (0.05 * 16) * (1 - math.exp(-0.05 * 16 * 200.0))
+    assert O.rel_l2(O.ifft(p, s.cwz), amp * np.sin(4 * y)) < 1e-12
+
+
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+def test_mass_conservation(model):
+    p = O.Params(model=model, Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, nu=5e-3, dt=5e-4, lam=1e-3,
+                 epsilon=0.3, sigma=0.1, sigma_12=0.1, sigma_13=0.08, sigma_23=0.06)
+    if model == "ternary":
+        c = O.init_noise_real(p, 4, 1 / 3, 0.05)
+    else:
+        c = O.init_noise_real(p, 4, 0.5, 0.1)
+    s0 = O.state_from_real(p, O.random_vorticity(p, 2), *c)
+    s = O.run(p, s0, 50)
+    for a, b in zip(s.fields()[1:], s0.fields()[1:]):
+        assert abs(a[0, 0] - b[0, 0]) <= 1e-13 * abs(b[0, 0])
+
+
+# ---------------------------------------------------------------- diagnostics / ICs
+def test_spectrum_sums_to_kinetic_energy():
+    p = O.Params(model="ternary", Nx=64, Ny=64, lx=TWO_PI, ly=TWO_PI)
+    c = O.init_noise_real(p, 4, 1 / 3, 0.05)
+    s = O.state_from_real(p, O.random_vorticity(p, 2), *c)
+    spec, _ = O.spectrum(p, s)
+    eu = O.kinetic_energy(p, s)[0]
+    assert math.isclose(float(np.sum(spec["ek"])), eu, rel_tol=1e-12)
+    mean_c1_sq = float(np.mean(O.ifft(p, O.truncation(p, s.cphi1)) ** 2))
+    assert math.isclose(float(np.sum(spec["sph1"])), mean_c1_sq, rel_tol=1e-12)
+
+
+def test_glibc_rand_known_values():
+    # glibc TYPE_3 generator: srand(1) -> 1804289383, 846930886 (widely published)
+    v = O.glibc_rand_stream(1, 2) * 2147483647.0
+    assert v[0] == 1804289383.0 and v[1] == 846930886.0
+
+
+def test_splitmix64_known_value():
+    # SplitMix64 with seed 0: first output 0xE220A8397B1DCDAF
+    u = O.splitmix64_uniform(0, 1)[0]
+    assert u == (0xE220A8397B1DCDAF >> 11) / 2.0 ** 53
