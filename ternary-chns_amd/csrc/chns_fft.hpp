@@ -77,7 +77,7 @@ template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) 
 // One Stockham stage of radix R on the cyclic registers v[8] of thread t.
 // Ns = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
-template <int LOG2N, int R, int NS, bool INV, bool LAST>
+template <int LOG2N, int R, int NS, bool INV, bool LAST, int S>
 __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / 8;
@@ -114,25 +114,27 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
     } else {
       const int base = (b / NS) * NS * R + k;
 #pragma unroll
-      for (int q = 0; q < R; ++q) lds[lp(base + q * NS)] = a[q];
+      for (int q = 0; q < R; ++q) lds[lp(base + q * NS) * S] = a[q];
     }
   }
 }
 
 __device__ __forceinline__ void lds_sync() { __syncthreads(); }
 
-template <int LOG2N>
+template <int LOG2N, int S>
 __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
   constexpr int T = (1 << LOG2N) / 8;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = lds[lp(t + m * T)];
+  for (int m = 0; m < 8; ++m) v[m] = lds[lp(t + m * T) * S];
 }
 
-// Full transform.  v: cyclic input -> cyclic output.  lds: LdsSize entries private
-// to this transform.  Starts and ends with a workgroup barrier-free state:
+// Full transform.  v: cyclic input -> cyclic output.  lds: this transform's
+// slot in a workgroup array holding S transforms interleaved (entry e of
+// transform c at lds_all[e*S + c]), so lanes of different transforms touch
+// adjacent addresses instead of aliasing banks.  Starts and ends with a workgroup barrier-free state:
 // the caller must synchronise before lds is reused by a different transform
 // (fft() itself begins with a barrier so back-to-back calls are safe).
-template <int LOG2N, bool INV>
+template <int LOG2N, bool INV, int S>
 __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   static_assert(LOG2N >= 4 && LOG2N <= 12, "supported lengths 16..4096");
   constexpr int NR8 = LOG2N / 3;
@@ -141,35 +143,35 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
   lds_sync();
   if constexpr (REM == 0) {
     // NR8 >= 2 here
-    stage<LOG2N, 8, 1, INV, false>(v, lds, tw, t);
-    lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
+    stage<LOG2N, 8, 1, INV, false, S>(v, lds, tw, t);
+    lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
     if constexpr (NR8 == 2) {
-      stage<LOG2N, 8, 8, INV, true>(v, lds, tw, t);
+      stage<LOG2N, 8, 8, INV, true, S>(v, lds, tw, t);
     } else if constexpr (NR8 == 3) {
-      stage<LOG2N, 8, 8, INV, false>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 64, INV, true>(v, lds, tw, t);
+      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
+      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
+      stage<LOG2N, 8, 64, INV, true, S>(v, lds, tw, t);
     } else {
-      stage<LOG2N, 8, 8, INV, false>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 64, INV, false>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 512, INV, true>(v, lds, tw, t);
+      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
+      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
+      stage<LOG2N, 8, 64, INV, false, S>(v, lds, tw, t);
+      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
+      stage<LOG2N, 8, 512, INV, true, S>(v, lds, tw, t);
     }
   } else {
     constexpr int RL = 1 << REM;
-    stage<LOG2N, 8, 1, INV, false>(v, lds, tw, t);
-    lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
+    stage<LOG2N, 8, 1, INV, false, S>(v, lds, tw, t);
+    lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
     if constexpr (NR8 >= 2) {
-      stage<LOG2N, 8, 8, INV, false>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
+      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
+      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
     }
     if constexpr (NR8 >= 3) {
-      stage<LOG2N, 8, 64, INV, false>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N>(v, lds, t); lds_sync();
+      stage<LOG2N, 8, 64, INV, false, S>(v, lds, tw, t);
+      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
     }
     constexpr int NSL = (NR8 == 1) ? 8 : (NR8 == 2 ? 64 : 512);
-    stage<LOG2N, RL, NSL, INV, true>(v, lds, tw, t);
+    stage<LOG2N, RL, NSL, INV, true, S>(v, lds, tw, t);
   }
 }
 

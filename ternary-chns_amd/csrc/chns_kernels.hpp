@@ -111,21 +111,21 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
 }
 
 // Split the forward transform Z of a + ib into A = F(a), B = F(b) for
-// n = 0..N/2 and hand them to op(n, A, B).  Uses lds (LdsSize entries).
-template <int LOG2N, class Op>
+// n = t + m*T <= N/2 and hand them to op(m, n, A, B).  Uses lds (LdsSize entries).
+template <int LOG2N, int S, class Op>
 __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / 8;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) lds[lp(t + m * T)] = v[m];
+  for (int m = 0; m < 8; ++m) lds[lp(t + m * T) * S] = v[m];
   lds_sync();
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
     const int n = t + m * T;
     if (n <= N / 2) {
-      const cd zn = lds[lp(n)];
-      const cd zr = lds[lp((N - n) & (N - 1))];
-      op(n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
+      const cd zn = lds[lp(n) * S];
+      const cd zr = lds[lp((N - n) & (N - 1)) * S];
+      op(m, n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
     }
   }
 }
@@ -140,12 +140,12 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colA(Bufs B, KParams P) {
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < P.Jt;
-  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + c;
   const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
   cd v[8];
   col_load_masked<LOG2N>(S, j, ok ? B.amask[j] : -1, t, v);
-  fft<LOG2N, true>(v, lds, B.tw, t);
+  fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
   col_store_rm<LOG2N>(H, P.ldJ, j, ok, t, v);
 }
 
@@ -158,12 +158,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + r;
   cd v[8];
   const cd* h1 = B.Hc1 + (size_t)x * P.ldJ;
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + (size_t)x * P.ldJ : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  fft<LOG2N, true>(v, lds, B.tw, t);
+  fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -186,10 +186,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-  fft<LOG2N, false>(v, lds, B.tw, t);
+  fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
   cd* o1 = B.Mh1 + (size_t)x * P.ldF;
   cd* o2 = B.Mh2 + (size_t)x * P.ldF;
-  row_split<LOG2N>(v, lds, t, [&](int n, cd a, cd b) {
+  row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
     o1[n] = a;
     if (MODEL == MODEL_TERNARY) o2[n] = b;
   });
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < P.Jt;
   const int a = ok ? B.amask[j] : -1;
-  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
   int g = blockIdx.y;
   if (MODEL == MODEL_NS) g += 2;                 // NS: only the velocity groups
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
     // mu_i = T(M_i + 0.75 g_i eps k^2 c_i)  (surface_tension.c:116-128)
     // binary: mu = T(1.5 sigma eps k^2 c + 24 sigma/eps M)  (surface_tension.c:139-142,166)
     col_load_rm<LOG2N>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
-    fft<LOG2N, false>(v, lds, B.tw, t);
+    fft<LOG2N, false, CF::C>(v, lds, B.tw, t);
     col_load_masked<LOG2N>(g == 0 ? B.C1 : B.C2, j, a, t, s);
     const double mc = g == 0 ? P.mc1 : P.mc2;
 #pragma unroll
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
         v[m] = mk(0.0, 0.0);
       }
     }
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
@@ -242,35 +242,35 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
       s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
       v[m] = ctimes_ik(s[m], ky);
     }
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
     col_load_masked<LOG2N>(B.W, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], ky);
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
     col_load_masked<LOG2N>(B.C1, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(B.Hdx1, P.ldJ, j, ok, t, v);
     if constexpr (MODEL == MODEL_TERNARY) {
       col_load_masked<LOG2N>(B.C2, j, a, t, s);
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-      fft<LOG2N, true>(v, lds, B.tw, t);
+      fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
       col_store_rm<LOG2N>(B.Hdx2, P.ldJ, j, ok, t, v);
     }
   }
@@ -285,105 +285,103 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + r;
   const size_t ro = (size_t)x * P.ldJ;
+  const size_t fo = (size_t)x * P.ldF;
   const double sc = P.scale;
   cd v[8];
-  double u[8], w[8], pw[8];
-  // u, v (nonlin.c:47-48)
-  row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-  fft<LOG2N, true>(v, lds, B.tw, t);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
-  // grad w; P = u wx + v wy (nonlin.c:62)
-  row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-  fft<LOG2N, true>(v, lds, B.tw, t);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) pw[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
-  const cd* G1o = nullptr;
-  (void)G1o;
   if constexpr (MODEL == MODEL_NS) {
+    double u[8], w[8];
+    // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
+    row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], 0.0);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o = B.G1 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd) { o[n] = mk(-A.x, -A.y); });  // nonlin.c:92
-  } else if constexpr (MODEL == MODEL_BINARY) {
-    double mu[8], fx[8], fy[8], adv[8];
-    row_load_pair<LOG2N, false>(B.Hmu1 + ro, nullptr, P.Jt, P.f, t, v);
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
+    row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) mu[m] = v[m].x * sc;
-    row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true>(v, lds, B.tw, t);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const double dx = v[m].x * sc, dy = v[m].y * sc;
-      fx[m] = dx * mu[m];                     // surface_tension.c:183-184
-      fy[m] = dy * mu[m];
-      adv[m] = u[m] * dx + w[m] * dy;         // nonlin.c:207
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], fx[m]);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o1 = B.G1 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bx) {
-      const cd iky = ctimes_ik(Bx, P.f * (double)n);
-      o1[n] = mk(-A.x - iky.x, -A.y - iky.y);
-    });
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], adv[m]);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o2 = B.G2 + (size_t)x * P.ldF;
-    cd* o3 = B.G3 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) { o2[n] = A; o3[n] = Bv; });
+    for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
+    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    cd* o = B.G1 + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[n] = mk(-A.x, -A.y); });
   } else {
-    double A1[8], A2[8], fx[8], fy[8], ad1[8];
-    row_load_pair<LOG2N, false>(B.Hmu1 + ro, B.Hmu2 + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    // The order keeps few real fields live: first the surface-tension force
+    // F = sum_i A_i grad c_i (surface_tension.c:138-220), transformed at once
+    // (F(Fx) kept in registers, F(Fy) stored), then the advection products.
+    constexpr bool TER = MODEL == MODEL_TERNARY;
+    double fx[8], fy[8], d1x[8], d1y[8], d2x[8], d2y[8];
+    {
+      double A1[8], A2[8];
+      row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
+      fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const double m1 = v[m].x * sc, m2 = v[m].y * sc;
+        if (TER) {
+          A1[m] = (P.A11 * m1 + P.A12 * m2);   // surface_tension.c:144
+          A2[m] = (P.A21 * m1 + P.A22 * m2);   // surface_tension.c:195
+        } else {
+          A1[m] = m1;                          // binary mu (surface_tension.c:183-184)
+        }
+      }
+      row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
+      fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        d1x[m] = v[m].x * sc;
+        d1y[m] = v[m].y * sc;
+        fx[m] = d1x[m] * A1[m];
+        fy[m] = d1y[m] * A1[m];
+      }
+      if constexpr (TER) {
+        row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
+        fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          d2x[m] = v[m].x * sc;
+          d2y[m] = v[m].y * sc;
+          fx[m] += d2x[m] * A2[m];
+          fy[m] += d2y[m] * A2[m];
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = mk(fx[m], fy[m]);
+    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    cd fxh[5];
+    cd* o2 = B.G2 + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) { fxh[m] = A; o2[n] = Bv; });
+    // advection: u.grad c_i (nonlin.c:266-348) and u.grad w (nonlin.c:56-62)
+    double u[8], w[8], ad1[8], ad2[8];
+    row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
-      const double m1 = v[m].x * sc, m2 = v[m].y * sc;
-      A1[m] = (P.A11 * m1 + P.A12 * m2);     // surface_tension.c:144
-      A2[m] = (P.A21 * m1 + P.A22 * m2);     // surface_tension.c:195
+      u[m] = v[m].x * sc;
+      w[m] = v[m].y * sc;
+      ad1[m] = d1x[m] * u[m] + d1y[m] * w[m];
+      if (TER) ad2[m] = d2x[m] * u[m] + d2y[m] * w[m];
     }
-    row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
+    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const double dx = v[m].x * sc, dy = v[m].y * sc;
-      fx[m] = dx * A1[m];
-      fy[m] = dy * A1[m];
-      ad1[m] = dx * u[m] + dy * w[m];          // nonlin.c:292-293
-    }
-    row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true>(v, lds, B.tw, t);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const double dx = v[m].x * sc, dy = v[m].y * sc;
-      fx[m] += dx * A2[m];
-      fy[m] += dy * A2[m];
-      A2[m] = dx * u[m] + dy * w[m];           // adv2 (nonlin.c:334-335), reuses A2's registers
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(pw[m], fx[m]);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o1 = B.G1 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bx) {
-      const cd iky = ctimes_ik(Bx, P.f * (double)n);
+    for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), ad1[m]);
+    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    // W1 = -F(u.grad w) - i ky F(Fx): the -ky part of curl (curl.c:26) applied row-side
+    cd* o1 = B.G1 + fo;
+    cd* o3 = B.G3 + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) {
+      const cd iky = ctimes_ik(fxh[m], P.f * (double)n);
       o1[n] = mk(-A.x - iky.x, -A.y - iky.y);
+      o3[n] = Bv;
     });
+    if constexpr (TER) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o2 = B.G2 + (size_t)x * P.ldF;
-    cd* o3 = B.G3 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) { o2[n] = A; o3[n] = Bv; });
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(A2[m], 0.0);
-    fft<LOG2N, false>(v, lds, B.tw, t);
-    cd* o4 = B.G4 + (size_t)x * P.ldF;
-    row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd) { o4[n] = A; });
+      for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
+      fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      cd* o4 = B.G4 + fo;
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[n] = A; });
+    }
   }
 }
 
@@ -397,18 +395,18 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < H;
-  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
   const int g = blockIdx.y;
   cd v[8], s[8];
   const cd* Gm = g == 0 ? B.G1 : (g == 1 ? B.G3 : B.G4);
   col_load_rm<LOG2N>(Gm, P.ldF, j, ok, t, v);
-  fft<LOG2N, false>(v, lds, B.tw, t);
+  fft<LOG2N, false, CF::C>(v, lds, B.tw, t);
   if (g == 0) {
     if constexpr (MODEL != MODEL_NS) {
       // + i kx F(Fy): the curl of the surface-tension force (curl.c:26)
       col_load_rm<LOG2N>(B.G2, P.ldF, j, ok, t, s);
-      fft<LOG2N, false>(s, lds, B.tw, t);
+      fft<LOG2N, false, CF::C>(s, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
     }
@@ -416,7 +414,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   } else {
     // N_c = -F(u.grad c) - (lam/g) k^2 M  (nonlin.c:369-370); binary: - lam*(24 s/e k^2 M) (:240,251)
     col_load_rm<LOG2N>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
-    fft<LOG2N, false>(s, lds, B.tw, t);
+    fft<LOG2N, false, CF::C>(s, lds, B.tw, t);
     const double lg = g == 1 ? P.lg1 : P.lg2;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -462,7 +460,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
     const int a = okA ? B.amask[j] : -1;
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
-    fft<LOG2N, true>(v, lds, B.tw, t);
+    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
     col_store_rm<LOG2N>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }
@@ -479,17 +477,17 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int x = blockIdx.x * RC::R + r;
-  cd* lds = lds_all + r * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + r;
   cd v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const size_t o = (size_t)x * N + t + m * T;
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
-  fft<LOG2N, false>(v, lds, tw, t);
+  fft<LOG2N, false, RC::R>(v, lds, tw, t);
   cd* pa = oa + (size_t)x * ldF;
   cd* pb = ob ? ob + (size_t)x * ldF : nullptr;
-  row_split<LOG2N>(v, lds, t, [&](int n, cd A, cd Bv) {
+  row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
     pa[n] = A;
     if (pb) pb[n] = Bv;
   });
@@ -508,10 +506,10 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_fwd(const cd* __restr
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < H;
-  cd* lds = lds_all + c * LdsSize<LOG2N>::value;
+  cd* lds = lds_all + c;
   cd v[8];
   col_load_rm<LOG2N>(in, ldF, j, ok, t, v);
-  fft<LOG2N, false>(v, lds, tw, t);
+  fft<LOG2N, false, CF::C>(v, lds, tw, t);
   if (ok) {
     const int a = TRUNC ? amask[j] : N;
 #pragma unroll
