@@ -68,14 +68,36 @@ def cpu_baseline(N, kw, seconds_cap=60.0):
             "sample": f"1 ternary ETD2RK step at {N}^2 (numpy oracle, pocketfft, 1 thread), {dt:.1f} s"}
 
 
+def copy_probe(nbytes: int, reps: int = 4):
+    """Device-to-device copies of a known size (hipMemcpy) for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run D2D copies of this many bytes")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
               "a single 2048^2 grid is not sharded across GPUs here", file=sys.stderr)

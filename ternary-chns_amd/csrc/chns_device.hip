@@ -7,12 +7,13 @@
 namespace chns {
 
 namespace {
-template <int L> constexpr int ceil_div(int a) { return (a + ColCfg<L>::C - 1) / ColCfg<L>::C; }
+template <int L, int PP = 8> constexpr int ceil_div(int a) { return (a + ColCfg<L, PP>::C - 1) / ColCfg<L, PP>::C; }
 
+int round16(int n) { return (n + 15) / 16 * 16; }
 int groupsC(int model) { return model == MODEL_TERNARY ? 5 : (model == MODEL_BINARY ? 4 : 2); }
 
 template <int L> int colA(const Bufs& B, const KParams& P, int nph, hipStream_t s) {
-  k_colA<L><<<dim3(ceil_div<L>(P.Jt), nph), ColCfg<L>::NT, 0, s>>>(B, P);
+  k_colA<L><<<dim3(ceil_div<L, 16>(P.Jt), nph), ColCfg<L, 16>::NT, 0, s>>>(B, P);
   return (int)hipGetLastError();
 }
 template <int L> int rowB(int model, const Bufs& B, const KParams& P, hipStream_t s) {
@@ -22,7 +23,7 @@ template <int L> int rowB(int model, const Bufs& B, const KParams& P, hipStream_
   return (int)hipGetLastError();
 }
 template <int L> int colC(int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  const dim3 g(ceil_div<L>(P.Jt), groupsC(model));
+  const dim3 g(round16(ceil_div<L>(P.Jt) * groupsC(model)));
   if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
   else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
   else k_colC<L, MODEL_NS><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
@@ -36,7 +37,7 @@ template <int L> int rowD(int model, const Bufs& B, const KParams& P, hipStream_
   return (int)hipGetLastError();
 }
 template <int L, int ST> int colE_st(int model, const Bufs& B, const KParams& P, int nf, hipStream_t s) {
-  const dim3 g(ceil_div<L>(P.N / 2 + 1), nf);
+  const dim3 g(round16(ceil_div<L>(P.N / 2 + 1) * nf));
   if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
   else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
   else k_colE<L, MODEL_NS, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
@@ -52,9 +53,9 @@ int row_r2c(const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int 
 }
 template <int L>
 int col_fwd(bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF, hipStream_t s) {
-  const dim3 g(ceil_div<L>((1 << L) / 2 + 1));
-  if (trunc) k_col_fwd<L, true><<<g, ColCfg<L>::NT, 0, s>>>(in, out, tw, amask, ldF);
-  else k_col_fwd<L, false><<<g, ColCfg<L>::NT, 0, s>>>(in, out, tw, amask, ldF);
+  const dim3 g(ceil_div<L, 16>((1 << L) / 2 + 1));
+  if (trunc) k_col_fwd<L, true><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ldF);
+  else k_col_fwd<L, false><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ldF);
   return (int)hipGetLastError();
 }
 }  // namespace
@@ -69,7 +70,6 @@ int col_fwd(bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, i
     case 9: { constexpr int L = 9; return CALL; }   \
     case 10: { constexpr int L = 10; return CALL; } \
     case 11: { constexpr int L = 11; return CALL; } \
-    case 12: { constexpr int L = 12; return CALL; } \
     default: return (int)hipErrorInvalidValue;      \
   }
 

@@ -1,14 +1,14 @@
 // FP64 Stockham FFT engine for gfx950, staged through LDS.
 //
-// One transform of length N = 2^LOG2N is carried by T = N/8 threads; thread t
-// holds the eight points t + m*T (m = 0..7) in registers ("cyclic"
-// distribution) both before and after the transform, so a c2r can feed a
-// pointwise stage and an r2c without any extra data movement.
+// One transform of length N = 2^LOG2N is carried by T = N/P threads (P = 8 or
+// 16 points per thread); thread t holds the points t + m*T (m < P) in
+// registers ("cyclic" distribution) both before and after the transform, so a
+// c2r can feed a pointwise stage and an r2c without extra data movement.
 //
-// Stages are radix 8 (registers) with a final radix 2 or 4 when LOG2N is not a
-// multiple of 3.  Between stages the values go through LDS: every thread
-// writes its butterfly outputs to their Stockham positions, the workgroup
-// synchronises, and every thread reads its cyclic set back.  The
+// Stages are radix P (registers) with one smaller final radix when LOG2N is
+// not a multiple of log2(P).  Between stages the values go through LDS: every
+// thread writes its butterfly outputs to their Stockham positions, the
+// workgroup synchronises, and every thread reads its cyclic set back.  The
 // synchronisations are workgroup-wide, so all threads of a workgroup must call
 // the same transforms in the same order (inputs of idle lanes are zero).
 //
@@ -29,6 +29,14 @@ __device__ __forceinline__ cd cscale(cd a, double s) { return mk(a.x * s, a.y * 
 __device__ __forceinline__ cd ctimes_ik(cd z, double k) { return mk(-k * z.y, k * z.x); }
 // multiply by -i (forward) or +i (inverse)
 template <bool INV> __device__ __forceinline__ cd rot90(cd a) { return INV ? mk(-a.y, a.x) : mk(a.y, -a.x); }
+// multiply by exp(-+ 2 pi i e/16) given (cos, sin) of 2 pi e/16
+template <bool INV> __device__ __forceinline__ cd wmul(cd a, double c, double s) {
+  return INV ? mk(a.x * c - a.y * s, a.x * s + a.y * c) : mk(a.x * c + a.y * s, a.y * c - a.x * s);
+}
+
+constexpr double kR = 0.70710678118654752440084436210485;   // cos(pi/4)
+constexpr double kC8 = 0.92387953251128675612818318939679;  // cos(pi/8)
+constexpr double kS8 = 0.38268343236508977172845998403040;  // sin(pi/8)
 
 template <bool INV> __device__ __forceinline__ void bfly2(cd& a, cd& b) {
   cd t = a; a = cadd(t, b); b = csub(t, b);
@@ -42,24 +50,50 @@ template <bool INV> __device__ __forceinline__ void dft4(cd& a0, cd& a1, cd& a2,
 }
 
 template <bool INV> __device__ __forceinline__ void dft8(cd* a) {
-  constexpr double R = 0.70710678118654752440084436210485;
   // radix-2 split into even/odd, then two radix-4
   cd e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
   cd o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
   dft4<INV>(e0, e1, e2, e3);
   dft4<INV>(o0, o1, o2, o3);
   // twiddles w8^q on odd outputs: w8 = exp(-+ i pi/4)
-  cd w1 = INV ? mk(R * (o1.x - o1.y), R * (o1.x + o1.y)) : mk(R * (o1.x + o1.y), R * (o1.y - o1.x));
+  cd w1 = INV ? mk(kR * (o1.x - o1.y), kR * (o1.x + o1.y)) : mk(kR * (o1.x + o1.y), kR * (o1.y - o1.x));
   cd w2 = rot90<INV>(o2);
-  cd w3 = INV ? mk(-R * (o3.x + o3.y), R * (o3.x - o3.y)) : mk(R * (o3.y - o3.x), -R * (o3.x + o3.y));
+  cd w3 = INV ? mk(-kR * (o3.x + o3.y), kR * (o3.x - o3.y)) : mk(kR * (o3.y - o3.x), -kR * (o3.x + o3.y));
   a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
   a[1] = cadd(e1, w1); a[5] = csub(e1, w1);
   a[2] = cadd(e2, w2); a[6] = csub(e2, w2);
   a[3] = cadd(e3, w3); a[7] = csub(e3, w3);
 }
 
+// 16-point DFT as 4 x 4: X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 x[4 n1 + n2] W4^(n1 k1)
+template <bool INV> __device__ __forceinline__ void dft16(cd* a) {
+#pragma unroll
+  for (int n2 = 0; n2 < 4; ++n2) dft4<INV>(a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12]);
+  // a[n2 + 4 k1] *= W16^(n2 k1)
+  a[5] = wmul<INV>(a[5], kC8, kS8);     // e = 1
+  a[6] = wmul<INV>(a[6], kR, kR);       // e = 2
+  a[7] = wmul<INV>(a[7], kS8, kC8);     // e = 3
+  a[9] = wmul<INV>(a[9], kR, kR);       // e = 2
+  a[10] = rot90<INV>(a[10]);            // e = 4
+  a[11] = wmul<INV>(a[11], -kR, kR);    // e = 6
+  a[13] = wmul<INV>(a[13], kS8, kC8);   // e = 3
+  a[14] = wmul<INV>(a[14], -kR, kR);    // e = 6
+  a[15] = wmul<INV>(a[15], -kC8, -kS8); // e = 9
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) dft4<INV>(a[4 * k1], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
+  // a[4 k1 + k2] holds X[k1 + 4 k2]: transpose
+  cd b[16];
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) b[k1 + 4 * k2] = a[4 * k1 + k2];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) a[k] = b[k];
+}
+
 template <int R, bool INV> __device__ __forceinline__ void dftR(cd* a) {
-  if constexpr (R == 8) dft8<INV>(a);
+  if constexpr (R == 16) dft16<INV>(a);
+  else if constexpr (R == 8) dft8<INV>(a);
   else if constexpr (R == 4) dft4<INV>(a[0], a[1], a[2], a[3]);
   else bfly2<INV>(a[0], a[1]);
 }
@@ -69,48 +103,62 @@ template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw
   return INV ? mk(w.x, -w.y) : w;
 }
 
-// LDS index with one padding element after every 8: the radix-8 Stockham
-// writes at stride 8 (128 B) would otherwise hit the same banks.
+// LDS index with one padding element after every 8: the Stockham writes at
+// stride R (>= 128 B) would otherwise hit the same banks.
 __device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
 template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + (1 << LOG2N) / 8; };
 
-// One Stockham stage of radix R on the cyclic registers v[8] of thread t.
-// Ns = product of the radices of earlier stages.  LAST: outputs stay in v in
+// One Stockham stage of radix R on the cyclic registers v[P] of thread t.
+// NS = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
-template <int LOG2N, int R, int NS, bool INV, bool LAST, int S>
+template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S>
 __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   constexpr int N = 1 << LOG2N;
-  constexpr int T = N / 8;
-  constexpr int P = 8 / R;            // butterflies per thread
+  constexpr int T = N / P;
+  constexpr int Q = P / R;            // butterflies per thread
   constexpr int TWS = N / (NS * R);   // twiddle index stride
 #pragma unroll
-  for (int u = 0; u < P; ++u) {
+  for (int u = 0; u < Q; ++u) {
     const int b = t + u * T;
     const int k = b % NS;
     cd a[R];
 #pragma unroll
-    for (int m = 0; m < R; ++m) a[m] = v[u + m * P];
+    for (int m = 0; m < R; ++m) a[m] = v[u + m * Q];
     if constexpr (NS > 1) {
-      // w^m for m < R from three table loads (w, w^2, w^4)
-      cd w[8];
-      w[1] = twid<INV>(tw, k * TWS);
+      // w^m for m < R from table loads of w, w^2, w^4, w^8, applied as they
+      // are formed so few powers are live at once
+      const cd w1 = twid<INV>(tw, k * TWS);
+      a[1] = cmulc(a[1], w1);
       if constexpr (R >= 4) {
-        w[2] = twid<INV>(tw, 2 * k * TWS);
-        w[3] = cmulc(w[1], w[2]);
+        const cd w2 = twid<INV>(tw, 2 * k * TWS);
+        const cd w3 = cmulc(w1, w2);
+        a[2] = cmulc(a[2], w2);
+        a[3] = cmulc(a[3], w3);
+        if constexpr (R >= 8) {
+          const cd w4 = twid<INV>(tw, 4 * k * TWS);
+          a[4] = cmulc(a[4], w4);
+          a[5] = cmulc(a[5], cmulc(w1, w4));
+          a[6] = cmulc(a[6], cmulc(w2, w4));
+          a[7] = cmulc(a[7], cmulc(w3, w4));
+          if constexpr (R == 16) {
+            const cd w8 = twid<INV>(tw, 8 * k * TWS);
+            const cd w12 = cmulc(w4, w8);
+            a[8] = cmulc(a[8], w8);
+            a[9] = cmulc(a[9], cmulc(w1, w8));
+            a[10] = cmulc(a[10], cmulc(w2, w8));
+            a[11] = cmulc(a[11], cmulc(w3, w8));
+            a[12] = cmulc(a[12], w12);
+            a[13] = cmulc(a[13], cmulc(w1, w12));
+            a[14] = cmulc(a[14], cmulc(w2, w12));
+            a[15] = cmulc(a[15], cmulc(w3, w12));
+          }
+        }
       }
-      if constexpr (R == 8) {
-        w[4] = twid<INV>(tw, 4 * k * TWS);
-        w[5] = cmulc(w[1], w[4]);
-        w[6] = cmulc(w[2], w[4]);
-        w[7] = cmulc(w[3], w[4]);
-      }
-#pragma unroll
-      for (int m = 1; m < R; ++m) a[m] = cmulc(a[m], w[m]);
     }
     dftR<R, INV>(a);
     if constexpr (LAST) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[u + q * P] = a[q];
+      for (int q = 0; q < R; ++q) v[u + q * Q] = a[q];
     } else {
       const int base = (b / NS) * NS * R + k;
 #pragma unroll
@@ -121,58 +169,38 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
 
 __device__ __forceinline__ void lds_sync() { __syncthreads(); }
 
-template <int LOG2N, int S>
+template <int LOG2N, int P, int S>
 __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
-  constexpr int T = (1 << LOG2N) / 8;
+  constexpr int T = (1 << LOG2N) / P;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = lds[lp(t + m * T) * S];
+  for (int m = 0; m < P; ++m) v[m] = lds[lp(t + m * T) * S];
 }
 
-// Full transform.  v: cyclic input -> cyclic output.  lds: this transform's
-// slot in a workgroup array holding S transforms interleaved (entry e of
-// transform c at lds_all[e*S + c]), so lanes of different transforms touch
-// adjacent addresses instead of aliasing banks.  Starts and ends with a workgroup barrier-free state:
-// the caller must synchronise before lds is reused by a different transform
-// (fft() itself begins with a barrier so back-to-back calls are safe).
-template <int LOG2N, bool INV, int S>
-__device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
-  static_assert(LOG2N >= 4 && LOG2N <= 12, "supported lengths 16..4096");
-  constexpr int NR8 = LOG2N / 3;
-  constexpr int REM = LOG2N % 3;
-  // stage 1 .. NR8 radix 8, then radix 2^REM
-  lds_sync();
-  if constexpr (REM == 0) {
-    // NR8 >= 2 here
-    stage<LOG2N, 8, 1, INV, false, S>(v, lds, tw, t);
-    lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-    if constexpr (NR8 == 2) {
-      stage<LOG2N, 8, 8, INV, true, S>(v, lds, tw, t);
-    } else if constexpr (NR8 == 3) {
-      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 64, INV, true, S>(v, lds, tw, t);
-    } else {
-      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 64, INV, false, S>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-      stage<LOG2N, 8, 512, INV, true, S>(v, lds, tw, t);
-    }
-  } else {
-    constexpr int RL = 1 << REM;
-    stage<LOG2N, 8, 1, INV, false, S>(v, lds, tw, t);
-    lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-    if constexpr (NR8 >= 2) {
-      stage<LOG2N, 8, 8, INV, false, S>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-    }
-    if constexpr (NR8 >= 3) {
-      stage<LOG2N, 8, 64, INV, false, S>(v, lds, tw, t);
-      lds_sync(); lds_read_cyclic<LOG2N, S>(v, lds, t); lds_sync();
-    }
-    constexpr int NSL = (NR8 == 1) ? 8 : (NR8 == 2 ? 64 : 512);
-    stage<LOG2N, RL, NSL, INV, true, S>(v, lds, tw, t);
+template <int LOG2N, int P, bool INV, int S, int NS>
+__device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+  constexpr int N = 1 << LOG2N;
+  constexpr int R = (N / NS) >= P ? P : (N / NS);
+  constexpr bool LAST = NS * R == N;
+  stage<LOG2N, P, R, NS, INV, LAST, S>(v, lds, tw, t);
+  if constexpr (!LAST) {
+    lds_sync();
+    lds_read_cyclic<LOG2N, P, S>(v, lds, t);
+    lds_sync();
+    fft_from<LOG2N, P, INV, S, NS * R>(v, lds, tw, t);
   }
+}
+
+// Full transform.  v: cyclic input -> cyclic output (P points per thread).
+// lds: this transform's slot in a workgroup array holding S transforms
+// interleaved (entry e of transform c at lds_all[e*S + c]), so lanes of
+// different transforms touch adjacent addresses instead of aliasing banks.
+// Begins with a barrier, so back-to-back calls sharing lds are safe.
+template <int LOG2N, int P, bool INV, int S>
+__device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+  static_assert(P == 8 || P == 16, "8 or 16 points per thread");
+  static_assert((1 << LOG2N) >= P * 2 && LOG2N <= 12, "supported lengths");
+  lds_sync();
+  fft_from<LOG2N, P, INV, S, 1>(v, lds, tw, t);
 }
 
 }  // namespace chns

@@ -2,8 +2,8 @@
 //
 // Layouts in HBM (N = Nx = Ny, Ny2 = N/2+1):
 //   spectral state / saved N_n : column-major  S[j*N + i]   (j = ky index, i = kx index)
-//   row-pass inputs  (H*)      : row-major     H[x*ldJ + j], j < Jt (dealiased columns)
-//   row-pass outputs (Mh*, G*) : row-major     G[x*ldF + j], j < Ny2
+//   row-pass inputs  (H*)      : row-pair tiled [N/2][ldJ][2], j < Jt (dealiased columns)
+//   row-pass outputs (Mh*, G*) : row-pair tiled [N/2][ldF][2], j < Ny2
 // Column kernels (x direction) read the state contiguously and touch the
 // row-major arrays in chunks of C consecutive columns; row kernels (y
 // direction) are contiguous everywhere.
@@ -43,43 +43,86 @@ template <int LOG2N> __device__ __forceinline__ double kxof(int i, double f) {
 }
 
 // columns per column-kernel workgroup
-template <int LOG2N> struct ColCfg {
+// PP = points per thread: 16 (radix-16 stages, two LDS exchanges at N = 2048)
+// suits kernels that hold one column array across a transform; 8 keeps the
+// registers of kernels holding two.  The workgroup holds C columns whose LDS
+// slots stay within 150 KB (one workgroup per CU at N = 2048) or 80 KB.
+template <int LOG2N, int PP = 8> struct ColCfg {
   static constexpr int N = 1 << LOG2N;
-  static constexpr int T = N / 8;
-  static constexpr int C = (LOG2N >= 12) ? 2 : 4;
+  static constexpr int P = (PP == 16 && LOG2N >= 8) ? 16 : 8;
+  static constexpr int T = N / P;
+  static constexpr int C0 = (256 / T) > 4 ? (256 / T) : 4;
+  // N = 2048 with 8 points: 2 columns x 256 threads keeps a 256-VGPR budget
+  static constexpr int C = (LOG2N >= 11 && P == 8) ? 2 : C0;
   static constexpr int NT = C * T;
 };
 // rows per row-kernel workgroup
 template <int LOG2N> struct RowCfg {
   static constexpr int N = 1 << LOG2N;
-  static constexpr int T = N / 8;
-  static constexpr int R0 = (256 / T) > 0 ? (256 / T) : 1;
+  static constexpr int P = 8;
+  static constexpr int T = N / P;
+  static constexpr int R0 = (256 / T) > 1 ? (256 / T) : 1;
   static constexpr int R = R0 < N ? R0 : N;
   static constexpr int NT = R * T;
 };
 
-// ---------------------------------------------------------------- column I/O
-template <int LOG2N>
-__device__ __forceinline__ void col_load_rm(const cd* __restrict__ A, int ld, int j, bool ok, int t, cd* v) {
-  constexpr int T = (1 << LOG2N) / 8;
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = ok ? A[(size_t)(t + m * T) * ld + j] : mk(0.0, 0.0);
+// Row handled by lane-row r of this workgroup.  With R >= 2 the two rows of
+// a tiled pair share the workgroup; with R == 1 rows 2p and 2p+1 go to blocks
+// b and b+8, which sit on the same XCD (blocks are dealt to the 8 XCDs
+// round-robin) and start together, so the second half-line read hits L2.
+template <int LOG2N> __device__ __forceinline__ int row_of(int r) {
+  if constexpr (RowCfg<LOG2N>::R >= 2) {
+    return blockIdx.x * RowCfg<LOG2N>::R + r;
+  } else {
+    const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
+    return 2 * (((idx >> 1) << 3) + xcd) + (idx & 1);
+  }
 }
-template <int LOG2N>
+
+// Row-pair tiled layout of the row-pass arrays: element (x, j) of a logical
+// [N][ld] array lives at ((x/2)*ld + j)*2 + x%2, so C consecutive columns of two
+// consecutive rows form one contiguous 32*C-byte chunk for the column kernels,
+// while a row kernel holding both rows of a pair still covers whole lines.
+__device__ __forceinline__ size_t tix(int x, int j, int ld) {
+  return (((size_t)(x >> 1) * ld + j) << 1) + (x & 1);
+}
+
+// Work item (tile, group) of a column kernel launched on a 1-D grid of
+// ceil(ntiles*ngroups/16)*16 blocks.  Items 2q and 2q+1 (neighbouring tiles)
+// go to blocks b and b+8, which sit on the same XCD and start together, so
+// the two halves of each 128-byte line of the tiled arrays meet in that L2.
+// Returns false for padding blocks (the whole workgroup exits at once).
+__device__ __forceinline__ bool col_item(int ntiles, int ngroups, int* tile, int* group) {
+  const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
+  const int item = 2 * (((idx >> 1) << 3) + xcd) + (idx & 1);
+  if (item >= ntiles * ngroups) return false;
+  *tile = item % ntiles;
+  *group = item / ntiles;
+  return true;
+}
+
+// ---------------------------------------------------------------- column I/O
+template <int LOG2N, int P>
+__device__ __forceinline__ void col_load_rm(const cd* __restrict__ A, int ld, int j, bool ok, int t, cd* v) {
+  constexpr int T = (1 << LOG2N) / P;
+#pragma unroll
+  for (int m = 0; m < P; ++m) v[m] = ok ? A[tix(t + m * T, j, ld)] : mk(0.0, 0.0);
+}
+template <int LOG2N, int P>
 __device__ __forceinline__ void col_store_rm(cd* __restrict__ A, int ld, int j, bool ok, int t, const cd* v) {
-  constexpr int T = (1 << LOG2N) / 8;
+  constexpr int T = (1 << LOG2N) / P;
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) A[(size_t)(t + m * T) * ld + j] = v[m];
+    for (int m = 0; m < P; ++m) A[tix(t + m * T, j, ld)] = v[m];
   }
 }
 // masked load of a state column: modes with |i| > a are truncated (functions.c:75-78)
-template <int LOG2N>
+template <int LOG2N, int P>
 __device__ __forceinline__ void col_load_masked(const cd* __restrict__ S, int j, int a, int t, cd* v) {
   constexpr int N = 1 << LOG2N;
-  constexpr int T = N / 8;
+  constexpr int T = N / P;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
+  for (int m = 0; m < P; ++m) {
     const int i = t + m * T;
     v[m] = (absidx<LOG2N>(i) <= a) ? S[(size_t)j * N + i] : mk(0.0, 0.0);
   }
@@ -100,8 +143,8 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
     const int k = n <= N / 2 ? n : N - n;
     cd a = mk(0.0, 0.0), b = mk(0.0, 0.0);
     if (k < Jt) {
-      a = X[k];
-      if (Y) b = Y[k];
+      a = X[2 * k];
+      if (Y) b = Y[2 * k];
       if (YIK) b = ctimes_ik(b, f * (double)k);
     }
     if (n == 0 || n == N / 2) v[m] = mk(a.x, b.x);
@@ -133,8 +176,8 @@ __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
 // ============================================================ pass A
 // grid: (ceil(Jt/C), nphase)
 template <int LOG2N>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colA(Bufs B, KParams P) {
-  using CF = ColCfg<LOG2N>;
+__global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Bufs B, KParams P) {
+  using CF = ColCfg<LOG2N, 16>;
   constexpr int N = CF::N;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
@@ -143,10 +186,10 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colA(Bufs B, KParams P) {
   cd* lds = lds_all + c;
   const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
-  cd v[8];
-  col_load_masked<LOG2N>(S, j, ok ? B.amask[j] : -1, t, v);
-  fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-  col_store_rm<LOG2N>(H, P.ldJ, j, ok, t, v);
+  cd v[CF::P];
+  col_load_masked<LOG2N, CF::P>(S, j, ok ? B.amask[j] : -1, t, v);
+  fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+  col_store_rm<LOG2N, CF::P>(H, P.ldJ, j, ok, t, v);
 }
 
 // ============================================================ pass B
@@ -156,14 +199,14 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
-  const int x = blockIdx.x * RC::R + r;
+  const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
+  const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
   cd v[8];
-  const cd* h1 = B.Hc1 + (size_t)x * P.ldJ;
-  const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + (size_t)x * P.ldJ : nullptr;
+  const cd* h1 = B.Hc1 + tix(x, 0, P.ldJ);
+  const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+  fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -186,41 +229,43 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-  fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
-  cd* o1 = B.Mh1 + (size_t)x * P.ldF;
-  cd* o2 = B.Mh2 + (size_t)x * P.ldF;
+  fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+  cd* o1 = B.Mh1 + tix(x, 0, P.ldF);
+  cd* o2 = B.Mh2 + tix(x, 0, P.ldF);
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
-    o1[n] = a;
-    if (MODEL == MODEL_TERNARY) o2[n] = b;
+    o1[2 * n] = a;
+    if (MODEL == MODEL_TERNARY) o2[2 * n] = b;
   });
 }
 
 // ============================================================ pass C
-// grid: (ceil(Jt/C), groups): ternary 5 groups, binary 4, NS 2.
+// grid: 1-D over (tile, group) items (col_item): ternary 5 groups, binary 4, NS 2.
 template <int LOG2N, int MODEL>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
-  const int j = blockIdx.x * CF::C + c;
+  constexpr int NG = MODEL == MODEL_TERNARY ? 5 : (MODEL == MODEL_BINARY ? 4 : 2);
+  int tile, g;
+  if (!col_item((P.Jt + CF::C - 1) / CF::C, NG, &tile, &g)) return;
+  const int j = tile * CF::C + c;
   const bool ok = j < P.Jt;
   const int a = ok ? B.amask[j] : -1;
   cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
-  int g = blockIdx.y;
   if (MODEL == MODEL_NS) g += 2;                 // NS: only the velocity groups
   if (MODEL == MODEL_BINARY && g >= 1) g += 1;   // binary: one mu group
-  cd v[8], s[8];
+  cd v[CF::P], s[CF::P];
   if (g <= 1) {
     // mu_i = T(M_i + 0.75 g_i eps k^2 c_i)  (surface_tension.c:116-128)
     // binary: mu = T(1.5 sigma eps k^2 c + 24 sigma/eps M)  (surface_tension.c:139-142,166)
-    col_load_rm<LOG2N>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
-    fft<LOG2N, false, CF::C>(v, lds, B.tw, t);
-    col_load_masked<LOG2N>(g == 0 ? B.C1 : B.C2, j, a, t, s);
+    col_load_rm<LOG2N, CF::P>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
+    fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
+    col_load_masked<LOG2N, CF::P>(g == 0 ? B.C1 : B.C2, j, a, t, s);
     const double mc = g == 0 ? P.mc1 : P.mc2;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
       const double ks = ksq(kxof<LOG2N>(i, P.f), ky);
       if (absidx<LOG2N>(i) <= a) {
@@ -230,48 +275,48 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
         v[m] = mk(0.0, 0.0);
       }
     }
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
-    col_load_masked<LOG2N>(B.W, j, a, t, s);
+    col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
       const double ks = ksq(kxof<LOG2N>(i, P.f), ky);
       s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
       v[m] = ctimes_ik(s[m], ky);
     }
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(B.Hu, P.ldJ, j, ok, t, v);
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(B.Hv, P.ldJ, j, ok, t, v);
+    for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
-    col_load_masked<LOG2N>(B.W, j, a, t, s);
+    col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(B.Hwx, P.ldJ, j, ok, t, v);
+    for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], ky);
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(B.Hwy, P.ldJ, j, ok, t, v);
+    for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], ky);
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
-    col_load_masked<LOG2N>(B.C1, j, a, t, s);
+    col_load_masked<LOG2N, CF::P>(B.C1, j, a, t, s);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(B.Hdx1, P.ldJ, j, ok, t, v);
+    for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(B.Hdx1, P.ldJ, j, ok, t, v);
     if constexpr (MODEL == MODEL_TERNARY) {
-      col_load_masked<LOG2N>(B.C2, j, a, t, s);
+      col_load_masked<LOG2N, CF::P>(B.C2, j, a, t, s);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-      fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-      col_store_rm<LOG2N>(B.Hdx2, P.ldJ, j, ok, t, v);
+      for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
+      fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+      col_store_rm<LOG2N, CF::P>(B.Hdx2, P.ldJ, j, ok, t, v);
     }
   }
 }
@@ -279,31 +324,31 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
 // ============================================================ pass D
 // grid: N / R
 template <int LOG2N, int MODEL>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
-  const int x = blockIdx.x * RC::R + r;
+  const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
+  const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
-  const size_t ro = (size_t)x * P.ldJ;
-  const size_t fo = (size_t)x * P.ldF;
+  const size_t ro = tix(x, 0, P.ldJ);   // row base in the tiled layout; element k at +2k
+  const size_t fo = tix(x, 0, P.ldF);
   const double sc = P.scale;
   cd v[8];
   if constexpr (MODEL == MODEL_NS) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
     cd* o = B.G1 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[n] = mk(-A.x, -A.y); });
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[2 * n] = mk(-A.x, -A.y); });
   } else {
     // The order keeps few real fields live: first the surface-tension force
     // F = sum_i A_i grad c_i (surface_tension.c:138-220), transformed at once
@@ -313,7 +358,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
     {
       double A1[8], A2[8];
       row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
-      fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+      fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -325,7 +370,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
         }
       }
       row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
-      fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+      fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         d1x[m] = v[m].x * sc;
@@ -335,7 +380,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
       }
       if constexpr (TER) {
         row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
-        fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+        fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           d2x[m] = v[m].x * sc;
@@ -347,14 +392,16 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(fx[m], fy[m]);
-    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
-    cd fxh[5];
+    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+    // F(Fx) waits in LDS (not registers) until F(u.grad w) is ready
+    __shared__ cd fxs[RC::R * 5 * T];
+    cd* fxh = fxs + (size_t)t * RC::R + r;
     cd* o2 = B.G2 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) { fxh[m] = A; o2[n] = Bv; });
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) { fxh[m * T * RC::R] = A; o2[2 * n] = Bv; });
     // advection: u.grad c_i (nonlin.c:266-348) and u.grad w (nonlin.c:56-62)
     double u[8], w[8], ad1[8], ad2[8];
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       u[m] = v[m].x * sc;
@@ -363,61 +410,63 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowD(Bufs B, KParams P) {
       if (TER) ad2[m] = d2x[m] * u[m] + d2y[m] * w[m];
     }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), ad1[m]);
-    fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
     // W1 = -F(u.grad w) - i ky F(Fx): the -ky part of curl (curl.c:26) applied row-side
     cd* o1 = B.G1 + fo;
     cd* o3 = B.G3 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) {
-      const cd iky = ctimes_ik(fxh[m], P.f * (double)n);
-      o1[n] = mk(-A.x - iky.x, -A.y - iky.y);
-      o3[n] = Bv;
+      const cd iky = ctimes_ik(fxh[m * T * RC::R], P.f * (double)n);
+      o1[2 * n] = mk(-A.x - iky.x, -A.y - iky.y);
+      o3[2 * n] = Bv;
     });
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
-      fft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
       cd* o4 = B.G4 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[n] = A; });
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[2 * n] = A; });
     }
   }
 }
 
 // ============================================================ pass E
-// grid: (ceil(Ny2/C), fields): omega, then phase field(s).
+// grid: 1-D over (tile, field) items (col_item): omega, then phase field(s).
 template <int LOG2N, int MODEL, int STAGE>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
-  const int j = blockIdx.x * CF::C + c;
+  constexpr int NG = MODEL == MODEL_TERNARY ? 3 : (MODEL == MODEL_BINARY ? 2 : 1);
+  int tile, g;
+  if (!col_item((H + CF::C - 1) / CF::C, NG, &tile, &g)) return;
+  const int j = tile * CF::C + c;
   const bool ok = j < H;
   cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
-  const int g = blockIdx.y;
-  cd v[8], s[8];
+  cd v[CF::P], s[CF::P];
   const cd* Gm = g == 0 ? B.G1 : (g == 1 ? B.G3 : B.G4);
-  col_load_rm<LOG2N>(Gm, P.ldF, j, ok, t, v);
-  fft<LOG2N, false, CF::C>(v, lds, B.tw, t);
+  col_load_rm<LOG2N, CF::P>(Gm, P.ldF, j, ok, t, v);
+  fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
   if (g == 0) {
     if constexpr (MODEL != MODEL_NS) {
       // + i kx F(Fy): the curl of the surface-tension force (curl.c:26)
-      col_load_rm<LOG2N>(B.G2, P.ldF, j, ok, t, s);
-      fft<LOG2N, false, CF::C>(s, lds, B.tw, t);
+      col_load_rm<LOG2N, CF::P>(B.G2, P.ldF, j, ok, t, s);
+      fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
+      for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
     }
     if (P.f_k > 0 && j == P.f_k && t == 0) v[0].y += P.force_im;   // Kolmogorov forcing at (0, f_k)
   } else {
     // N_c = -F(u.grad c) - (lam/g) k^2 M  (nonlin.c:369-370); binary: - lam*(24 s/e k^2 M) (:240,251)
-    col_load_rm<LOG2N>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
-    fft<LOG2N, false, CF::C>(s, lds, B.tw, t);
+    col_load_rm<LOG2N, CF::P>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
+    fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
     const double lg = g == 1 ? P.lg1 : P.lg2;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < CF::P; ++m) {
       const double ks = ksq(kxof<LOG2N>(t + m * T, P.f), ky);
       if (MODEL == MODEL_TERNARY) {
         v[m] = csub(mk(-v[m].x, -v[m].y), cscale(s[m], lg * ks));
@@ -431,7 +480,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   cd* Nsv = g == 0 ? B.NW : (g == 1 ? B.NC1 : B.NC2);
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
       const size_t idx = (size_t)j * N + i;
       const int ti = j * H + absidx<LOG2N>(i);
@@ -455,13 +504,13 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
     }
   }
   // next stage's pass A for the phase fields
-  if (g >= 1 && (int)(blockIdx.x * CF::C) < P.Jt) {
+  if (g >= 1 && tile * CF::C < P.Jt) {
     const bool okA = j < P.Jt;
     const int a = okA ? B.amask[j] : -1;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
-    fft<LOG2N, true, CF::C>(v, lds, B.tw, t);
-    col_store_rm<LOG2N>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
+    for (int m = 0; m < CF::P; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
+    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }
 
@@ -475,8 +524,8 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
-  const int r = threadIdx.x / T, t = threadIdx.x % T;
-  const int x = blockIdx.x * RC::R + r;
+  const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
+  const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
   cd v[8];
 #pragma unroll
@@ -484,12 +533,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
     const size_t o = (size_t)x * N + t + m * T;
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
-  fft<LOG2N, false, RC::R>(v, lds, tw, t);
-  cd* pa = oa + (size_t)x * ldF;
-  cd* pb = ob ? ob + (size_t)x * ldF : nullptr;
+  fft<LOG2N, RC::P, false, RC::R>(v, lds, tw, t);
+  cd* pa = oa + tix(x, 0, ldF);
+  cd* pb = ob ? ob + tix(x, 0, ldF) : nullptr;
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
-    pa[n] = A;
-    if (pb) pb[n] = Bv;
+    pa[2 * n] = A;
+    if (pb) pb[2 * n] = Bv;
   });
 }
 
@@ -497,23 +546,23 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
 // with truncation() applied when TRUNC (init.c:256-262,291-294).
 // grid: (ceil(Ny2/C), 1)
 template <int LOG2N, bool TRUNC>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_fwd(const cd* __restrict__ in, cd* out,
+__global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* __restrict__ in, cd* out,
                                                               const cd* __restrict__ tw,
                                                               const int* __restrict__ amask, int ldF) {
-  using CF = ColCfg<LOG2N>;
+  using CF = ColCfg<LOG2N, 16>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < H;
   cd* lds = lds_all + c;
-  cd v[8];
-  col_load_rm<LOG2N>(in, ldF, j, ok, t, v);
-  fft<LOG2N, false, CF::C>(v, lds, tw, t);
+  cd v[CF::P];
+  col_load_rm<LOG2N, CF::P>(in, ldF, j, ok, t, v);
+  fft<LOG2N, CF::P, false, CF::C>(v, lds, tw, t);
   if (ok) {
     const int a = TRUNC ? amask[j] : N;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
       out[(size_t)j * N + i] = (absidx<LOG2N>(i) <= a) ? v[m] : mk(0.0, 0.0);
     }

@@ -54,7 +54,7 @@ const char* check_params(const chns_params* p) {
   if (p->Nx != p->Ny) return "this build requires Nx == Ny";
   if (p->lx != p->ly) return "this build requires lx == ly";
   const int l = ilog2(p->Nx);
-  if (l < 4 || l > 12) return "Nx must be a power of two in [16, 4096]";
+  if (l < 4 || l > 11) return "Nx must be a power of two in [16, 2048]";
   if (!(p->lx > 0.0)) return "lx must be positive";
   if (!(p->dt > 0.0)) return "dt must be positive";
   if (p->smooth_filter) return "SMOOTH_FILTER is not supported by this build";
