@@ -195,23 +195,28 @@ extern "C" int chns_create(chns_ctx** out, const chns_params* p, int device) {
   for (auto q : rf)
     if ((rc = dalloc(c, q, RF))) return bail(rc);
 
-  std::vector<double> tw(2 * (size_t)N), t1(4 * (size_t)H * H), t2(2 * (size_t)H * H);
+  const size_t HH = (size_t)H * H;
+  std::vector<double> tw(2 * (size_t)N), tw1(2 * HH), tc1(2 * HH), tw2(HH), tc2(HH);
   chns_host::build_twiddles(N, tw.data());
-  chns_host::build_coef_tables(&c->p, t1.data(), t2.data());
-  chns::cd* dtw;
-  double *dt1, *dt2;
+  chns_host::build_coef_tables(&c->p, tw1.data(), tc1.data(), tw2.data(), tc2.data());
+  chns::cd *dtw, *dw1, *dc1;
+  double *dw2, *dc2;
   int* dam;
-  if ((rc = dalloc(c, &dtw, N)) || (rc = dalloc(c, &dt1, t1.size())) || (rc = dalloc(c, &dt2, t2.size())) ||
-      (rc = dalloc(c, &dam, H)))
+  if ((rc = dalloc(c, &dtw, N)) || (rc = dalloc(c, &dw1, HH)) || (rc = dalloc(c, &dc1, HH)) ||
+      (rc = dalloc(c, &dw2, HH)) || (rc = dalloc(c, &dc2, HH)) || (rc = dalloc(c, &dam, H)))
     return bail(rc);
   if ((e = hipMemcpy(dtw, tw.data(), tw.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(dt1, t1.data(), t1.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(dt2, t2.data(), t2.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dw1, tw1.data(), tw1.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dc1, tc1.data(), tc1.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dw2, tw2.data(), tw2.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dc2, tc2.data(), tc2.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(dam, c->amask.data(), H * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return bail(hip_fail(c, e, "hipMemcpy(tables)"));
   b.tw = dtw;
-  b.tab1 = (const chns::coef4*)dt1;
-  b.tab2 = (const chns::coef2*)dt2;
+  b.tabw1 = dw1;
+  b.tabc1 = dc1;
+  b.tabw2 = dw2;
+  b.tabc2 = dc2;
   b.amask = dam;
   *out = c;
   return CHNS_OK;

@@ -111,7 +111,7 @@ template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) 
 // One Stockham stage of radix R on the cyclic registers v[P] of thread t.
 // NS = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
-template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S>
+template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S, bool PRUNED>
 __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / P;
@@ -124,6 +124,12 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
     cd a[R];
 #pragma unroll
     for (int m = 0; m < R; ++m) a[m] = v[u + m * Q];
+    if constexpr (PRUNED && NS == 1 && R == P) {
+      // dealiased input: points t + m*T with P/4 < m < 3P/4 lie in N/4 < n < 3N/4,
+      // where every truncated spectrum is zero
+#pragma unroll
+      for (int m = P / 4 + 1; m < 3 * P / 4; ++m) a[m] = mk(0.0, 0.0);
+    }
     if constexpr (NS > 1) {
       // w^m for m < R from table loads of w, w^2, w^4, w^8, applied as they
       // are formed so few powers are live at once
@@ -176,17 +182,17 @@ __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
   for (int m = 0; m < P; ++m) v[m] = lds[lp(t + m * T) * S];
 }
 
-template <int LOG2N, int P, bool INV, int S, int NS>
+template <int LOG2N, int P, bool INV, int S, int NS, bool PRUNED>
 __device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   constexpr int N = 1 << LOG2N;
   constexpr int R = (N / NS) >= P ? P : (N / NS);
   constexpr bool LAST = NS * R == N;
-  stage<LOG2N, P, R, NS, INV, LAST, S>(v, lds, tw, t);
+  stage<LOG2N, P, R, NS, INV, LAST, S, PRUNED && NS == 1>(v, lds, tw, t);
   if constexpr (!LAST) {
     lds_sync();
     lds_read_cyclic<LOG2N, P, S>(v, lds, t);
     lds_sync();
-    fft_from<LOG2N, P, INV, S, NS * R>(v, lds, tw, t);
+    fft_from<LOG2N, P, INV, S, NS * R, false>(v, lds, tw, t);
   }
 }
 
@@ -195,12 +201,14 @@ __device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ 
 // interleaved (entry e of transform c at lds_all[e*S + c]), so lanes of
 // different transforms touch adjacent addresses instead of aliasing banks.
 // Begins with a barrier, so back-to-back calls sharing lds are safe.
-template <int LOG2N, int P, bool INV, int S>
+// PRUNED: the input is a dealiased (truncated) spectrum, zero for indices
+// N/4 < n < 3N/4; the first stage skips those points.
+template <int LOG2N, int P, bool INV, int S, bool PRUNED = false>
 __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   static_assert(P == 8 || P == 16, "8 or 16 points per thread");
   static_assert((1 << LOG2N) >= P * 2 && LOG2N <= 12, "supported lengths");
   lds_sync();
-  fft_from<LOG2N, P, INV, S, 1>(v, lds, tw, t);
+  fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t);
 }
 
 }  // namespace chns

@@ -14,9 +14,9 @@ struct Derived {
 
 const char* check_params(const chns_params* p);
 void derive(const chns_params* p, Derived* d);
-// tab1: H*H entries {eta, z1w, beta, z1c}; tab2: H*H entries {z2w, z2c};
-// entry (j, |i|) at j*H + |i|.
-void build_coef_tables(const chns_params* p, double* tab1, double* tab2);
+// Entry (j, |i|) at j*H + |i|: tw1 {eta, z1w} and tc1 {beta, z1c} pairs,
+// tw2 z2w and tc2 z2c scalars.
+void build_coef_tables(const chns_params* p, double* tw1, double* tc1, double* tw2, double* tc2);
 void build_twiddles(int N, double* tw);
 void dealias_extent(const chns_params* p, int* amax);
 

@@ -132,9 +132,9 @@ __device__ __forceinline__ void col_load_masked(const cd* __restrict__ S, int j,
 // Assemble Z = X + iY (length N, Hermitian halves) from two dealiased
 // half-spectra rows so one complex c2r yields two real rows.  Only real parts
 // are used at ky = 0 and ky = N/2, as a c2r does.  YIK: Y is i*ky*Yrow.
+// Row entries X[st*k], Y[st*k] for k < Jt (Y may be null: zero).
 template <int LOG2N, bool YIK>
-__device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd* __restrict__ Y, int Jt,
-                                              double f, int t, cd* v) {
+__device__ __forceinline__ void assemble_pair(const cd* X, const cd* Y, int st, int Jt, double f, int t, cd* v) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / 8;
 #pragma unroll
@@ -143,8 +143,8 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
     const int k = n <= N / 2 ? n : N - n;
     cd a = mk(0.0, 0.0), b = mk(0.0, 0.0);
     if (k < Jt) {
-      a = X[2 * k];
-      if (Y) b = Y[2 * k];
+      a = X[st * k];
+      if (Y) b = Y[st * k];
       if (YIK) b = ctimes_ik(b, f * (double)k);
     }
     if (n == 0 || n == N / 2) v[m] = mk(a.x, b.x);
@@ -152,6 +152,49 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
     else v[m] = mk(a.x + b.y, b.x - a.y);
   }
 }
+
+// Pair from the row-pair tiled arrays in global memory (row element k at +2k).
+template <int LOG2N, bool YIK>
+__device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd* __restrict__ Y, int Jt,
+                                              double f, int t, cd* v) {
+  assemble_pair<LOG2N, YIK>(X, Y, 2, Jt, f, t, v);
+}
+
+// Streams the dealiased rows of the next field pair into LDS while the
+// current pair is transformed (one row per workgroup, T threads): gfx950
+// global_load_lds_dwordx4 copies 16 bytes per lane straight into LDS, lane l
+// of a wave into slot base + l of the staging area (base from M0, uniform).
+template <int LOG2N>
+struct RowStager {
+  static constexpr int N = 1 << LOG2N, T = N / 8, JMAX = N / 4 + 1;
+  cd* stg;   // 2*JMAX slots: X in [0, Jt), Y in [Jt, 2 Jt)
+  int Jt;
+  template <bool HASY>
+  __device__ __forceinline__ void issue(const cd* X, const cd* Y) const {
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int nslot = HASY ? 2 * Jt : Jt;
+    for (int base = wave * 64; base < nslot; base += T) {
+      const int s = base + lane;
+      if (s < nslot) {
+        const cd* src = X + 2 * s;
+        if constexpr (HASY) {
+          if (s >= Jt) src = Y + 2 * (s - Jt);
+        }
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(stg + base), 16, 0, 0);
+      }
+    }
+  }
+  // this thread's copies have landed (vmcnt(0)), then every wave's
+  __device__ __forceinline__ void wait() const {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+  }
+  template <bool YIK, bool HASY>
+  __device__ __forceinline__ void assemble(double f, int t, cd* v) const {
+    assemble_pair<LOG2N, YIK>(stg, HASY ? stg + Jt : nullptr, 1, Jt, f, t, v);
+  }
+};
 
 // Split the forward transform Z of a + ib into A = F(a), B = F(b) for
 // n = t + m*T <= N/2 and hand them to op(m, n, A, B).  Uses lds (LdsSize entries).
@@ -188,7 +231,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Bufs B, KParam
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
   cd v[CF::P];
   col_load_masked<LOG2N, CF::P>(S, j, ok ? B.amask[j] : -1, t, v);
-  fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+  fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
   col_store_rm<LOG2N, CF::P>(H, P.ldJ, j, ok, t, v);
 }
 
@@ -206,7 +249,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
   const cd* h1 = B.Hc1 + tix(x, 0, P.ldJ);
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+  fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -275,7 +318,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
         v[m] = mk(0.0, 0.0);
       }
     }
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
@@ -287,35 +330,35 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
       s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
       v[m] = ctimes_ik(s[m], ky);
     }
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
     col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], ky);
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
     col_load_masked<LOG2N, CF::P>(B.C1, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(B.Hdx1, P.ldJ, j, ok, t, v);
     if constexpr (MODEL == MODEL_TERNARY) {
       col_load_masked<LOG2N, CF::P>(B.C2, j, a, t, s);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-      fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+      fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
       col_store_rm<LOG2N, CF::P>(B.Hdx2, P.ldJ, j, ok, t, v);
     }
   }
@@ -339,11 +382,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
     fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
@@ -354,11 +397,26 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
     // F = sum_i A_i grad c_i (surface_tension.c:138-220), transformed at once
     // (F(Fx) kept in registers, F(Fy) stored), then the advection products.
     constexpr bool TER = MODEL == MODEL_TERNARY;
+    // One row per workgroup (N = 2048): each field pair is staged in LDS
+    // while the previous pair is transformed.
+    constexpr bool STG = RC::R == 1;
+    __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+    const RowStager<LOG2N> st{stg_all, P.Jt};
+    if constexpr (STG) {
+      st.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
+      st.wait();
+    }
     double fx[8], fy[8], d1x[8], d1y[8], d2x[8], d2y[8];
     {
       double A1[8], A2[8];
-      row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
-      fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+      if constexpr (STG) {
+        st.template assemble<false, TER>(P.f, t, v);
+        lds_sync();
+        st.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+      } else {
+        row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
+      }
+      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -369,8 +427,16 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
           A1[m] = m1;                          // binary mu (surface_tension.c:183-184)
         }
       }
-      row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
-      fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+      if constexpr (STG) {
+        st.wait();
+        st.template assemble<true, true>(P.f, t, v);
+        lds_sync();
+        if (TER) st.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
+        else st.template issue<true>(B.Hu + ro, B.Hv + ro);
+      } else {
+        row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
+      }
+      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         d1x[m] = v[m].x * sc;
@@ -379,8 +445,15 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
         fy[m] = d1y[m] * A1[m];
       }
       if constexpr (TER) {
-        row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
-        fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+        if constexpr (STG) {
+          st.wait();
+          st.template assemble<true, true>(P.f, t, v);
+          lds_sync();
+          st.template issue<true>(B.Hu + ro, B.Hv + ro);
+        } else {
+          row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
+        }
+        fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           d2x[m] = v[m].x * sc;
@@ -400,8 +473,15 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
     row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) { fxh[m * T * RC::R] = A; o2[2 * n] = Bv; });
     // advection: u.grad c_i (nonlin.c:266-348) and u.grad w (nonlin.c:56-62)
     double u[8], w[8], ad1[8], ad2[8];
-    row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+    if constexpr (STG) {
+      st.wait();
+      st.template assemble<false, true>(P.f, t, v);
+      lds_sync();
+      st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
+    } else {
+      row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+    }
+    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
       u[m] = v[m].x * sc;
@@ -409,8 +489,13 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
       ad1[m] = d1x[m] * u[m] + d1y[m] * w[m];
       if (TER) ad2[m] = d2x[m] * u[m] + d2y[m] * w[m];
     }
-    row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R>(v, lds, B.tw, t);
+    if constexpr (STG) {
+      st.wait();
+      st.template assemble<false, true>(P.f, t, v);
+    } else {
+      row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
+    }
+    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), ad1[m]);
     fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
@@ -486,16 +571,14 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
       const int ti = j * H + absidx<LOG2N>(i);
       const cd u0 = S[idx];
       if constexpr (STAGE == 1) {
-        const coef4 k4 = B.tab1[ti];
-        const double e = g == 0 ? k4.eta : k4.beta;
-        const double z = g == 0 ? k4.z1w : k4.z1c;
+        const cd k1 = (g == 0 ? B.tabw1 : B.tabc1)[ti];
+        const double e = k1.x, z = k1.y;
         const cd a = cadd(cscale(u0, e), cscale(v[m], z));
         S[idx] = a;
         Nsv[idx] = v[m];
         s[m] = a;
       } else {
-        const coef2 k2 = B.tab2[ti];
-        const double z = g == 0 ? k2.z2w : k2.z2c;
+        const double z = (g == 0 ? B.tabw2 : B.tabc2)[ti];
         const cd nn = Nsv[idx];
         const cd a = cadd(u0, cscale(csub(v[m], nn), z));
         S[idx] = a;
@@ -509,7 +592,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
     const int a = okA ? B.amask[j] : -1;
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
-    fft<LOG2N, CF::P, true, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }

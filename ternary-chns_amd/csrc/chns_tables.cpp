@@ -136,7 +136,7 @@ static void etd_mode(const chns_params* p, const Derived& d, double ksqr, double
   }
 }
 
-void build_coef_tables(const chns_params* p, double* tab1, double* tab2) {
+void build_coef_tables(const chns_params* p, double* tw1, double* tc1, double* tw2, double* tc2) {
   Derived d;
   derive(p, &d);
   const int H = d.H;
@@ -146,10 +146,11 @@ void build_coef_tables(const chns_params* p, double* tab1, double* tab2) {
       const double ksqr = kx * kx + ky * ky;
       double e = 0, z1w = 0, z2w = 0, b = 0, z1c = 0, z2c = 0;
       etd_mode(p, d, ksqr, &e, &z1w, &z2w, &b, &z1c, &z2c);
-      double* t1 = tab1 + 4 * ((size_t)j * H + ii);
-      t1[0] = e; t1[1] = z1w; t1[2] = b; t1[3] = z1c;
-      double* t2 = tab2 + 2 * ((size_t)j * H + ii);
-      t2[0] = z2w; t2[1] = z2c;
+      const size_t o = (size_t)j * H + ii;
+      tw1[2 * o] = e; tw1[2 * o + 1] = z1w;
+      tc1[2 * o] = b; tc1[2 * o + 1] = z1c;
+      tw2[o] = z2w;
+      tc2[o] = z2c;
     }
   }
 }

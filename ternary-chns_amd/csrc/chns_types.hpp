@@ -5,8 +5,6 @@ namespace chns {
 
 struct cd { double x, y; };   // complex double, same layout as double[2]
 
-struct alignas(16) coef4 { double eta, z1w, beta, z1c; };   // stage-1 ETD2RK coefficients
-struct alignas(16) coef2 { double z2w, z2c; };              // stage-2 ETD2RK coefficients
 
 struct KParams {
   int N;         // grid size, Nx = Ny = N
@@ -36,8 +34,12 @@ struct Bufs {
   cd *Mh1, *Mh2;               // row-major [N][ldF]
   cd *G1, *G2, *G3, *G4;       // W1, Fy, adv1, adv2 (row-major [N][ldF])
   const cd* tw;                // twiddles exp(-2 pi i n/N)
-  const coef4* tab1;           // [(N/2+1)*(N/2+1)] indexed j*(N/2+1) + |i|
-  const coef2* tab2;
+  // ETD2RK coefficients, [(N/2+1)*(N/2+1)] indexed j*(N/2+1) + |i|, one
+  // table per field kind so each field group reads only its own
+  const cd* tabw1;             // omega stage 1: {eta, z1}
+  const cd* tabc1;             // phase stage 1: {beta, z1}
+  const double* tabw2;         // omega stage 2: z2
+  const double* tabc2;         // phase stage 2: z2
   const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
 };
 
