@@ -81,6 +81,33 @@ int chns_sync(chns_ctx *ctx);
  * spectrum() (spectrum.c:22-27) called at main.c:226,281-284. */
 int chns_truncate_state(chns_ctx *ctx);
 
+/* ---- diagnostics (device computation, fixed-order reductions) ---------- */
+
+/* Bulk and gradient free energy of the untruncated phase fields
+ * (free_energy.c:13-157); BINARY and TERNARY only. */
+int chns_free_energy(chns_ctx *ctx, double *ephi, double *egphi);
+/* Mean kinetic energy eu and the per-phase kinetic energies averaged over
+ * the points with c_i >= 0.5 (kinetic_energy.c:3-144); eph2 is ternary only,
+ * eph1/eph2 untouched for NS.  Any output may be NULL. */
+int chns_kinetic_energy(chns_ctx *ctx, double *eu, double *eph1, double *eph2);
+/* Interface counts of the phase fields thresholded at 0.5 (perimeter.c:127-238):
+ * counts[0] sign changes to the x+1/y+1 neighbour and counts[1] points inside
+ * for phase 1, counts[2..3] for phase 2 (ternary).  The reference prints
+ * Sp_t = count*dx, Sp_0 = 2 sqrt(pi inside dx dy), def = Sp_t/Sp_0 - 1. */
+int chns_perimeter(chns_ctx *ctx, double *counts);
+/* Shell spectra of spectrum.c:16-121: ek, ensk and the phase spectra sph1
+ * (binary: sph), sph2, sp12 have nshell entries; ekx, eky have Nx/2+2 (the
+ * reference's kxm-sized arrays are too short for factor <= 1).  NULL skips an
+ * output.  truncate_state != 0 also applies spectrum()'s truncation of the
+ * phase fields to the state (spectrum.c:22-27). */
+int chns_spectrum(chns_ctx *ctx, int truncate_state, double *ek, double *ensk, double *sph1, double *sph2,
+                  double *sp12, double *ekx, double *eky);
+/* Packed real [Nx][Ny] field which = 0 (vorticity), 1, 2 (phase fields),
+ * truncated first when truncate != 0 (the DUMP_REAL_MPI path, main.c:332-358). */
+int chns_real_field(chns_ctx *ctx, int which, int truncate, double *out);
+/* Lengths of the spectrum outputs (nshell = round(Nx/sqrt(2) + 1), main.c:60). */
+int chns_diag_sizes(const chns_params *p, int *nshell, int *nxy);
+
 /* Per-pass kernel timing with HIP events recorded on the context's stream
  * around every launch (measurement aid, off by default; no reference
  * counterpart - the reference only times file output with MPI_Wtime,

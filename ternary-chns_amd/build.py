@@ -39,6 +39,7 @@ def build(verbose: bool = False) -> str:
     objs = {
         "tables": os.path.join(BUILD, "chns_tables.o"),
         "api": os.path.join(BUILD, "chns_api.o"),
+        "diag": os.path.join(BUILD, "chns_api_diag.o"),
         "device": os.path.join(BUILD, "chns_device.o"),
     }
     steps = [
@@ -46,13 +47,15 @@ def build(verbose: bool = False) -> str:
          "-c", os.path.join(CSRC, "chns_tables.cpp"), "-o", objs["tables"]],
         [HIPCC, "-O2", "-fPIC", "-std=c++17", "-I" + INC, "-I/opt/rocm/include", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
          "-c", os.path.join(CSRC, "chns_api.cpp"), "-o", objs["api"]],
+        [HIPCC, "-O2", "-fPIC", "-std=c++17", "-I" + INC, "-I/opt/rocm/include", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+         "-c", os.path.join(CSRC, "chns_api_diag.cpp"), "-o", objs["diag"]],
         [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
          "-c", os.path.join(CSRC, "chns_device.hip"), "-o", objs["device"]],
     ]
-    with ThreadPoolExecutor(3) as ex:
+    with ThreadPoolExecutor(4) as ex:
         list(ex.map(_run, steps))
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["api"],
-          objs["tables"]])
+          objs["diag"], objs["tables"]])
     if verbose:
         print("built", OUT)
     return OUT

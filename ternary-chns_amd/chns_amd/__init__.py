@@ -30,6 +30,8 @@ EXPORTS = (
     "chns_params_default", "chns_create", "chns_destroy", "chns_last_error", "chns_set_state",
     "chns_get_state", "chns_set_real_state", "chns_step", "chns_sync", "chns_truncate_state",
     "chns_etd_tables", "chns_dealias_extent", "chns_timing_enable", "chns_timing_read",
+    "chns_free_energy", "chns_kinetic_energy", "chns_perimeter", "chns_spectrum", "chns_real_field",
+    "chns_diag_sizes",
 )
 PASS_NAMES = ("colA", "rowB", "colC", "rowD", "colE1", "colE2")
 
@@ -82,6 +84,12 @@ def load_library(path: str = LIB_PATH):
     lib.chns_dealias_extent.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     lib.chns_timing_enable.argtypes = [V, I]
     lib.chns_timing_read.argtypes = [V, D, ctypes.POINTER(ctypes.c_int)]
+    lib.chns_free_energy.argtypes = [V, D, D]
+    lib.chns_kinetic_energy.argtypes = [V, D, D, D]
+    lib.chns_perimeter.argtypes = [V, D]
+    lib.chns_spectrum.argtypes = [V, I, D, D, D, D, D, D, D]
+    lib.chns_real_field.argtypes = [V, I, I, D]
+    lib.chns_diag_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     _lib = lib
     return lib
 
@@ -186,6 +194,50 @@ class Context:
 
     def truncate_state(self):
         self._check(self.lib.chns_truncate_state(self.h), "chns_truncate_state")
+
+    # -- diagnostics (free_energy.c, kinetic_energy.c, perimeter.c, spectrum.c)
+    def free_energy(self):
+        """(ephi, egphi) of the untruncated phase fields (free_energy.c:13-157)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.chns_free_energy(self.h, ctypes.byref(a), ctypes.byref(b)), "chns_free_energy")
+        return a.value, b.value
+
+    def kinetic_energy(self):
+        """(eu,), (eu, eph) or (eu, eph1, eph2) as kinetic_energy.c prints them."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.chns_kinetic_energy(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "chns_kinetic_energy")
+        return (a.value, b.value, c.value)[: 1 + self.nph]
+
+    def perimeter(self):
+        """[(interface, inside) per phase field] (perimeter.c:127-238)."""
+        q = np.zeros(4)
+        self._check(self.lib.chns_perimeter(self.h, _dp(q)), "chns_perimeter")
+        return [(int(q[2 * f]), int(q[2 * f + 1])) for f in range(self.nph)]
+
+    def spectrum(self, truncate_state: bool = True):
+        """Shell spectra of spectrum.c; truncates the phase fields like spectrum() does."""
+        ns, nxy = ctypes.c_int(), ctypes.c_int()
+        self.lib.chns_diag_sizes(ctypes.byref(self.p), ctypes.byref(ns), ctypes.byref(nxy))
+        out = {k: np.zeros(ns.value) for k in ("ek", "ensk", "s1", "s2", "s12")}
+        out["ekx"] = np.zeros(nxy.value)
+        out["eky"] = np.zeros(nxy.value)
+        self._check(self.lib.chns_spectrum(self.h, int(bool(truncate_state)), *[_dp(out[k]) for k in
+                    ("ek", "ensk", "s1", "s2", "s12", "ekx", "eky")]), "chns_spectrum")
+        if self.nph == 1:
+            out["sph"] = out.pop("s1")
+            del out["s2"], out["s12"]
+        elif self.nph == 2:
+            out["sph1"], out["sph2"], out["sp12"] = out.pop("s1"), out.pop("s2"), out.pop("s12")
+        else:
+            del out["s1"], out["s2"], out["s12"]
+        return out
+
+    def real_field(self, which: int, truncate: bool = True):
+        """Packed real [Nx][Ny] field: 0 vorticity, 1/2 phase fields (main.c:332-358)."""
+        out = np.empty((self.Nx, self.Ny))
+        self._check(self.lib.chns_real_field(self.h, int(which), int(bool(truncate)), _dp(out)), "chns_real_field")
+        return out
 
     # -- measurement ------------------------------------------------------
     def timing(self, enable: bool = True):

@@ -12,35 +12,32 @@
 #include "chns_launch.hpp"
 #include "chns_types.hpp"
 
-struct chns_ctx {
-  chns_params p{};
-  chns_host::Derived d{};
-  int dev = 0;
-  hipStream_t stream = nullptr;
-  int Jt = 0, ldJ = 0, ldF = 0, nph = 0;
-  chns::KParams kp{};
-  chns::Bufs b{};
-  std::vector<void*> allocs;
-  std::vector<int> amask;
-  bool have_state = false, hc_valid = false;
-  std::string err;
-  // event timing (chns_timing_enable)
-  bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<int> ev_kind;   // one entry per recorded pair
-  size_t ev_used = 0;
-};
+#include "chns_ctx.hpp"
 
-namespace {
-
-int fail(chns_ctx* c, int code, const std::string& msg) {
+int chns_fail(chns_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
 
-int hip_fail(chns_ctx* c, hipError_t e, const char* what) {
-  return fail(c, CHNS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what) {
+  return chns_fail(c, CHNS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return chns_hip_fail(c, e, "hipMalloc");
+  c->allocs.push_back(q);
+  e = hipMemset(q, 0, bytes);
+  if (e != hipSuccess) return chns_hip_fail(c, e, "hipMemset");
+  *p = q;
+  return 0;
+}
+
+namespace {
+
+int fail(chns_ctx* c, int code, const std::string& msg) { return chns_fail(c, code, msg); }
+int hip_fail(chns_ctx* c, hipError_t e, const char* what) { return chns_hip_fail(c, e, what); }
 
 #define CHNS_HIP(ctx, call)                                          \
   do {                                                               \
@@ -57,14 +54,7 @@ int hip_fail(chns_ctx* c, hipError_t e, const char* what) {
 int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 template <class T> int dalloc(chns_ctx* c, T** p, size_t count) {
-  void* q = nullptr;
-  hipError_t e = hipMalloc(&q, count * sizeof(T));
-  if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
-  c->allocs.push_back(q);
-  e = hipMemset(q, 0, count * sizeof(T));
-  if (e != hipSuccess) return hip_fail(c, e, "hipMemset");
-  *p = (T*)q;
-  return 0;
+  return chns_dalloc_bytes(c, (void**)p, count * sizeof(T));
 }
 
 // host IDXC [N][H] complex <-> device column-major [H][N]

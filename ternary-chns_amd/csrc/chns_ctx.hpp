@@ -1,0 +1,38 @@
+// Context of the C ABI (internal to the library).
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "chns.h"
+#include "chns_host.hpp"
+#include "chns_types.hpp"
+
+struct chns_ctx {
+  chns_params p{};
+  chns_host::Derived d{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int Jt = 0, ldJ = 0, ldF = 0, nph = 0;
+  chns::KParams kp{};
+  chns::Bufs b{};
+  std::vector<void*> allocs;
+  std::vector<int> amask;
+  bool have_state = false, hc_valid = false;
+  std::string err;
+  // event timing (chns_timing_enable)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_kind;   // one entry per recorded pair
+  size_t ev_used = 0;
+  // diagnostics scratch (allocated on first use)
+  chns::cd *dT1 = nullptr, *dT2 = nullptr;   // tiled [N][ldF] column-iFFT outputs
+  double* dred = nullptr;                    // reduction partials + result
+  double* dspec = nullptr;                   // spectrum row partials
+  double* dspec_out = nullptr;               // spectrum sums
+};
+
+int chns_fail(chns_ctx* c, int code, const std::string& msg);
+int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what);
+int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes);

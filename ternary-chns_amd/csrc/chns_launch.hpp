@@ -19,4 +19,15 @@ int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, c
                    hipStream_t s);
 int launch_truncate(cd* S, const int* amask, int N, hipStream_t s);
 int col_tile(int log2n);   // columns per column-kernel workgroup
+
+// diagnostics (chns_diag.hpp)
+struct RedArgs;
+struct SpecArgs;
+int launch_col_inv_full(int log2n, int op, bool trunc, const cd* S, cd* out, const cd* tw, const int* amask,
+                        int ldF, double f, hipStream_t s);
+int launch_row_c2r_full(int log2n, const cd* X, const cd* Y, double* oa, double* ob, const cd* tw, int ldF,
+                        double scale, hipStream_t s);
+int launch_reduce(int kind, const RedArgs& A, double* partial, double* out, hipStream_t s);
+int reduce_blocks(int N);
+int launch_spectrum(const SpecArgs& A, double* out, double* outy, double* outx, hipStream_t s);
 }  // namespace chns

@@ -43,4 +43,28 @@ struct Bufs {
   const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
 };
 
+// ---- diagnostics (chns_diag.hpp)
+// per-mode operator of the full column iFFT
+enum { COP_ID = 0, COP_IKX = 1, COP_IKY = 2, COP_U = 3, COP_V = 4 };
+// reduction kinds (up to RED_Q sums per kind)
+constexpr int RED_Q = 5;
+enum { RED_FE_TER = 0, RED_FE_BIN = 1, RED_KE_TER = 2, RED_KE_BIN = 3, RED_KE_NS = 4, RED_PERIM = 5 };
+
+struct RedArgs {
+  const double* f[6];   // packed [N][N] fields (meaning per kind)
+  double g1, g2, g3;    // gammas (ternary)
+  int N;
+};
+
+// spectrum.c binning arguments
+struct SpecArgs {
+  const cd *W, *C1, *C2;   // column-major state
+  const int* amask;
+  double f, dk, kymax, scale2;
+  int N, nsh, nxy, nph;
+  double* part;    // [5][N][nsh]
+  double* party;   // [N][nxy]
+  double* partx;   // [N]
+};
+
 }  // namespace chns
