@@ -466,6 +466,56 @@ def run(p: Params, s: State, nsteps: int, spectrum_every: int = 0):
     return s
 
 
+def main_loop(p: Params, s: State, maxiter: int, nfile: int, diag_every: int = 1):
+    """The time loop of main.c:214-367 with its diagnostics and dumps.
+
+    Returns the records the reference writes, in order: ``ken`` rows
+    (t, eu[, eph1[, eph2]]) of kinetic_energy.c:109,135,141; ``fe`` rows
+    (t, ephi, egphi) of free_energy.c:149; ``gamma`` rows per phase
+    (t, Sp_0, Sp_t, def) of perimeter.c:117-123,253-265; ``spectra`` (one dict
+    per spectrum() call, spectrum.c:150-187) and ``dumps`` (real fields
+    [wz, phi1, phi2] per dump, main.c:230-250 at t = 0 without truncation,
+    main.c:330-357 every navg steps with truncation).  ``diag_every`` > 1
+    thins the per-step diagnostics (1 = the reference).
+    """
+    navg = maxiter // nfile                                          # main.c:52
+    tb = etd_tables(p)
+    nph = {MODEL_NS: 0, MODEL_BINARY: 1, MODEL_TERNARY: 2}[p.model]
+    rec = {"ken": [], "fe": [], "gamma": [[] for _ in range(nph)], "spectra": [], "dumps": []}
+    t = 0.0
+
+    def diagnostics(s):
+        if nph:
+            rec["fe"].append((t,) + tuple(free_energy(p, s)))
+        rec["ken"].append((t,) + tuple(kinetic_energy(p, s)))
+        if nph:
+            for ph, (c1, c2) in enumerate(perimeter(p, s)):
+                spt = np.float64(c1 * p.dx)
+                sp0 = np.float64(2.0 * math.sqrt(PI * c2 * p.dx * p.dy))
+                with np.errstate(divide="ignore", invalid="ignore"):   # no phase-1 points: 0/0 as in C
+                    rec["gamma"][ph].append((t, float(sp0), float(spt), float((spt / sp0) - 1.0)))
+
+    def dump(s, trunc):
+        f = [s.cwz] + ([s.cphi1] if nph >= 1 else []) + ([s.cphi2] if nph == 2 else [])
+        rec["dumps"].append([ifft(p, truncation(p, c) if trunc else c) for c in f])
+
+    diagnostics(s)
+    sp, s = spectrum(p, s)
+    rec["spectra"].append(sp)
+    dump(s, False)
+    for it in range(1, maxiter + 1):                                 # main.c:260
+        t += p.dt                                                    # :263
+        s = etd2rk(p, s, tb)
+        if it % diag_every == 0:
+            diagnostics(s)
+        if it % (navg // 2) == 0:                                    # :281
+            sp, s = spectrum(p, s)
+            rec["spectra"].append(sp)
+        if it % navg == 0:                                           # :301
+            dump(s, True)
+    return rec, s
+
+
 def truncate_state(p: Params, s: State):
     """State side effect of spectrum(): truncation of the phase fields (spectrum.c:22-27)."""
     return State(s.cwz,
