@@ -21,8 +21,9 @@ reference's operation order with IEEE double arithmetic and the C library's
 round-off (numpy/pocketfft FFTs instead of FFTW).
 
 Pinning (what checks each function independently of this restatement) is
-listed in ``PINS`` at the bottom of the file; the tests in
-``tests/test_oracle_*.py`` implement them.
+listed in ``PINS`` at the bottom of the file; ``tests/test_oracle_pins.py``
+implements most of them.  Frozen outputs at N = 16 and 32 are committed under
+``tests/golden/`` (written by ``tests/golden/make_golden.py``).
 """
 from __future__ import annotations
 
@@ -778,12 +779,22 @@ def rel_l2(a, b):
 
 
 PINS = {
-    "dfft/ifft": "numpy.fft vs direct O(N^4) DFT, Parseval, single modes (tests/test_oracle_fft.py)",
-    "kxky/truncation": "integer lattice count of kept modes (tests/test_oracle_spectral.py)",
+    # function: the independent check (tests/test_oracle_pins.py unless named)
+    "dfft/ifft": "direct O(N^4) DFT, Parseval, single-mode transforms",
+    "kxky/truncation": "Nyquist sign; kept-mode count equals the integer lattice count",
     "get_velocity/curl": "curl(get_velocity(w)) = w, div u = 0, single-mode closed form",
     "potential": "exact rational evaluation (fractions) + pure-phase / symmetric zeros",
-    "surface_tension": "real-space analytic curl(sum A_i grad c_i) with sympy for band-limited fields; 1-D zero",
-    "nonlin": "Taylor-Green / single-mode zero advection; analytic u.grad c and lap(muF) (sympy)",
-    "etd_tables": "bit-compared to an independent C evaluation compiled with gcc",
+    "surface_tension": "sympy real-space closed form for band-limited fields; 1-D fields give zero",
+    "nonlin": "sympy closed form of u.grad w, u.grad c and lap(muF); Taylor-Green zero advection",
+    "etd_tables": "closed form where well conditioned; bit-compared with the C tables (tests/test_abi.py)",
     "etd2rk": "Stokes decay, forced Kolmogorov steady state, mass conservation",
+    "free_energy": "binary closed form; d/dh F = <mu, dc> against the chemical potential; "
+                   "ternary reduces to binary on two-phase states",
+    "kinetic_energy": "single-mode closed form A^2/(4k^2), per-phase means",
+    "spectrum": "Parseval (sum ek = eu, sum sph1 = <c1^2>); single-mode phase spectrum",
+    "perimeter": "band of two straight interfaces: Sp_t = 2L, half the points inside",
+    "init_ternary_noise_real": "glibc rand known values; bit-compared with the C driver (tests/test_driver.py)",
+    "init_binary_droplet_real": "0.5 level on the circle r = pi/4.5; compared with the C driver",
+    "splitmix64_uniform": "published first output for seed 0",
+    "main_loop": "cadence and record counts against the C driver's output files (tests/test_driver.py)",
 }

@@ -307,3 +307,102 @@ def test_splitmix64_known_value():
     # SplitMix64 with seed 0: first output 0xE220A8397B1DCDAF
     u = O.splitmix64_uniform(0, 1)[0]
     assert u == (0xE220A8397B1DCDAF >> 11) / 2.0 ** 53
+
+
+def test_kinetic_energy_single_mode_closed_form():
+    # omega = A cos(3x): v = (A/3) sin(3x), u = 0, so <|u|^2/2> = A^2/36 everywhere;
+    # the phase masks depend on y only, so each per-phase mean equals the global one
+    p = O.Params(model="ternary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    x, y = grid(p)
+    A = 2.0
+    s = O.state_from_real(p, A * np.cos(3 * x), 0.5 + 0.4 * np.cos(y - 0.1), 0.5 + 0.3 * np.sin(2 * y + 0.2))
+    eu, e1, e2 = O.kinetic_energy(p, s)
+    for v in (eu, e1, e2):
+        assert math.isclose(v, A * A / 36.0, rel_tol=1e-13)
+
+
+def _binary_mu(p, c):
+    """Variational derivative of the binary free energy density:
+    24 (sigma/eps) c(1-c)(1-2c) - 1.5 sigma eps lap(c), lap by exact spectral derivative."""
+    k2 = O.ksqr_grid(p)
+    lap = np.fft.irfft2(-k2 * np.fft.rfft2(c), s=c.shape)
+    return 24.0 * (p.sigma / p.epsilon) * c * (1 - c) * (1 - 2 * c) - 1.5 * p.sigma * p.epsilon * lap
+
+
+def test_binary_free_energy_variation_is_chemical_potential():
+    # d/dh F[c + h dc] = <mu, dc>: ties free_energy.c to the chemical potential of
+    # surface_tension.c:20-46 (a dropped or mis-scaled term in either breaks it)
+    p = O.Params(model="binary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, sigma=0.3, epsilon=0.4)
+    x, y = grid(p)
+    c = 0.5 + 0.3 * np.cos(x) * np.sin(2 * y) + 0.1 * np.sin(3 * x)
+    dc = 0.2 * np.cos(x) * np.sin(2 * y) + 0.1 * np.sin(3 * x) + 0.05 * np.cos(2 * x + y)
+
+    def F(h):
+        s = O.State(np.zeros((32, 17), complex), O.dfft(c + h * dc))
+        return sum(O.free_energy(p, s))
+
+    h = 1e-4
+    dF = (8 * (F(h) - F(-h)) - (F(2 * h) - F(-2 * h))) / (12 * h)   # 4th-order central difference
+    assert math.isclose(dF, float(np.mean(_binary_mu(p, c) * dc)), rel_tol=1e-9)
+
+
+def test_ternary_free_energy_reduces_to_binary_on_a_two_phase_state():
+    # with c2 = 0 the ternary energy is the binary one with sigma = sigma_13 (gamma1+gamma3
+    # = 2 sigma_13); with c1 = 0 it is the binary one with sigma = sigma_23
+    N, L = 32, TWO_PI
+    pt = O.Params(model="ternary", Nx=N, Ny=N, lx=L, ly=L, sigma_12=0.3, sigma_13=0.2, sigma_23=0.45, epsilon=0.3)
+    x, y = grid(pt)
+    c = 0.5 + 0.3 * np.cos(x) * np.sin(2 * y) + 0.1 * np.sin(3 * x)
+    z = np.zeros_like(c)
+    cw = np.zeros((N, N // 2 + 1), complex)
+    for sig, c1, c2 in ((pt.sigma_13, c, z), (pt.sigma_23, z, c)):
+        pb = O.Params(model="binary", Nx=N, Ny=N, lx=L, ly=L, sigma=sig, epsilon=0.3)
+        et = O.free_energy(pt, O.State(cw, O.dfft(c1), O.dfft(c2)))
+        eb = O.free_energy(pb, O.State(cw, O.dfft(c)))
+        assert math.isclose(et[0], eb[0], rel_tol=1e-12) and math.isclose(et[1], eb[1], rel_tol=1e-12)
+
+
+def test_binary_free_energy_closed_form():
+    # c = a + b cos(2y): <c^2 (1-c)^2> from an exact integral, <|grad c|^2> = b^2 m^2 / 2
+    p = O.Params(model="binary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, sigma=0.1)
+    a, b, m = sp.Rational(1, 2), sp.Rational(3, 10), 2
+    t = sp.symbols("t")
+    cc = a + b * sp.cos(m * t)
+    bulk = float(sp.integrate(sp.expand(cc ** 2 * (1 - cc) ** 2), (t, 0, 2 * sp.pi)) / (2 * sp.pi))
+    x, y = grid(p)
+    s = O.State(np.zeros((32, 17), complex), O.dfft(0.5 + 0.3 * np.cos(2 * y)))
+    ephi, egphi = O.free_energy(p, s)
+    assert math.isclose(ephi, 12 * (p.sigma / p.epsilon) * bulk, rel_tol=1e-13)
+    assert math.isclose(egphi, 0.75 * p.sigma * p.epsilon * 0.09 * 4 / 2, rel_tol=1e-13)
+
+
+def test_perimeter_of_a_band_is_twice_the_box_length():
+    # c >= 0.5 on a band in y: two straight interfaces of length L each, so the
+    # printed Sp_t = count * dx equals 2 L and half the points lie inside
+    for N in (32, 64):
+        p = O.Params(model="binary", Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI)
+        x, y = grid(p)
+        s = O.State(np.zeros((N, N // 2 + 1), complex), O.dfft(0.5 + 0.4 * np.cos(y - 0.1)))
+        ((edge, inside),) = O.perimeter(p, s)
+        assert math.isclose(edge * p.dx, 2 * TWO_PI, rel_tol=1e-14)
+        assert inside == N * N // 2
+
+
+def test_phase_spectrum_of_a_single_mode():
+    # c = a + b cos(3y): sph holds a^2 at shell 0 and b^2/2 (the mean of (b cos)^2) at shell 3
+    p = O.Params(model="binary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
+    x, y = grid(p)
+    s = O.State(np.zeros((32, 17), complex), O.dfft(0.5 + 0.3 * np.cos(3 * y)))
+    spec, _ = O.spectrum(p, s)
+    expect = np.zeros(p.nshell)
+    expect[0], expect[3] = 0.25, 0.045
+    np.testing.assert_allclose(spec["sph"], expect, rtol=1e-13, atol=1e-18)
+
+
+def test_binary_droplet_half_level_is_the_circle():
+    p = O.Params(model="binary", Nx=64, Ny=64)
+    c = O.init_binary_droplet_real(p)
+    x, y = grid(p)
+    r = np.sqrt((x - p.lx / 2) ** 2 + (y - p.ly / 2) ** 2)
+    assert np.all((c > 0.5) == (r < math.pi / 4.5))
+    assert c.max() <= 1.0 and c.min() >= 0.0
