@@ -6,18 +6,29 @@ Prints one JSON line.  ``value`` is grid-point steps/s (N^2 * steps/s) with
 the state resident in HBM; the timed region is exactly K steps bracketed by
 device synchronisations of the library's stream.  ``roofline`` reports the
 dominant pass: algorithmic bytes per launch (the compulsory traffic of the
-five-pass design, coefficient tables excluded) over its mean launch duration
-from HIP events recorded on the library's stream during the timed region.
+five-pass design, coefficient tables excluded) over its mean launch duration,
+from HIP events recorded on the library's stream around every launch during K
+further steps right after the timed region (events between launches cost
+about 7% of the step, so the timed region itself carries none).
 ``cpu_baseline`` times the numpy oracle (one core, pocketfft) on one step of
-the same workload.
+the same workload.  ``roofline.traffic`` is the HBM bytes per launch of that
+pass from two child ``rocprofv3 --pmc`` runs of this script (FETCH_SIZE, then
+WRITE_SIZE, KiB per dispatch), each scaled by the factor that makes the same
+counter read the true size of a plain 256 MiB device-to-device copy
+(``--copy-probe``); null when rocprofv3 is unavailable or ``--no-traffic``.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import math
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -25,6 +36,50 @@ sys.path.insert(0, os.path.join(ROOT, "ternary-chns_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# counter scale factors: a 256 MiB hipMemcpy D2D (__amd_rocclr_copyBuffer) reads
+# FETCH_SIZE = 131084 KiB and WRITE_SIZE = 262144 KiB on MI355X (ROCm 7.2)
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
+
+
+def kernel_tag(pass_name: str, N: int) -> str:
+    """Substring of the demangled kernel name of a pass (ternary model)."""
+    L = int(round(math.log2(N)))
+    return {"colA": f"k_colA<{L}>", "rowB": f"k_rowB<{L}, 2>", "colC": f"k_colC<{L}, 2>",
+            "rowD": f"k_rowD<{L}, 2>", "colE1": f"k_colE<{L}, 2, 1>", "colE2": f"k_colE<{L}, 2, 2>"}[pass_name]
+
+
+def measure_traffic(N: int, tag: str, timeout: int = 120):
+    """Mean scaled FETCH_SIZE + WRITE_SIZE bytes per dispatch of the kernels whose
+    name contains ``tag``, from two child rocprofv3 runs of a 3-step bench."""
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="chns_pmc_")
+        try:
+            cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+                   sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1",
+                   "--N", str(N), "--no-cpu-baseline", "--no-traffic"]
+            try:
+                subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            vals = []
+            for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+                with open(f) as fh:
+                    for r in csv.DictReader(fh):
+                        if tag in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                            vals.append(float(r["Counter_Value"]))
+            if not vals:
+                return None, f"no {counter} records for {tag}"
+            res[counter] = sum(vals) / len(vals) * 1024.0
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+    fetch = res["FETCH_SIZE"] * FETCH_SCALE
+    write = res["WRITE_SIZE"] * WRITE_SCALE
+    return {"fetch": fetch, "write": write, "total": fetch + write}, None
 
 
 def workload(N=2048):
@@ -51,21 +106,26 @@ def pass_bytes(N, Jt, K):
     }, 42 * F + 48 * Ft + 6 * Fk
 
 
-def cpu_baseline(N, kw, seconds_cap=60.0):
-    import numpy as np  # noqa: F401
-
+def cpu_baseline(N, kw, seconds=10.0, max_steps=50):
+    """The numpy oracle (one thread, pocketfft) stepping the same workload until
+    ``seconds`` of CPU work have elapsed."""
     import oracle as O
 
     op = O.Params(model="ternary", **kw)
     c1, c2 = O.init_lens_real(op)
     s = O.state_from_real(op, None, c1, c2)
     tb = O.etd_tables(op)
+    steps = 0
     t0 = time.perf_counter()
-    s = O.etd2rk(op, s, tb)
+    while steps < max_steps:
+        s = O.etd2rk(op, s, tb)
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
     dt = time.perf_counter() - t0
-    steps = 1
     return {"value": N * N * steps / dt, "unit": "grid-pt steps/s", "cores": 1, "kind": "port",
-            "sample": f"1 ternary ETD2RK step at {N}^2 (numpy oracle, pocketfft, 1 thread), {dt:.1f} s"}
+            "sample": f"{steps} ternary ETD2RK steps of the {N}^2 lens workload (numpy oracle, pocketfft, "
+                      f"1 thread), {dt:.1f} s"}
 
 
 def copy_probe(nbytes: int, reps: int = 4):
@@ -94,6 +154,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
+    ap.add_argument("--traffic-pass", default="rowD", help="pass whose HBM traffic is counted")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
@@ -103,12 +165,15 @@ def main():
               "a single 2048^2 grid is not sharded across GPUs here", file=sys.stderr)
         sys.exit(2)
 
+    N = a.N
+    # counter runs first, in child processes, before this process touches the GPU
+    traffic, traffic_err = (None, "skipped") if a.no_traffic else measure_traffic(N, kernel_tag(a.traffic_pass, N))
+
     import numpy as np
 
     import chns_amd as C
     from chns_amd import ic
 
-    N = a.N
     kw = workload(N)
     p = C.params("ternary", **kw)
     amask = C.dealias_extent(p)
@@ -119,13 +184,16 @@ def main():
         ctx.set_real_state(None, c1, c2)
         ctx.step(a.warmup)
         ctx.sync()
-        ctx.timing_read()
-        ctx.timing(True)
-        ctx.sync()
+        # timed region: exactly K steps, no events between the launches
         t0 = time.perf_counter()
         ctx.step(a.steps)
         ctx.sync()
         t1 = time.perf_counter()
+        # per-pass kernel durations: HIP events around every launch on the
+        # library's stream over K more steps of the same run
+        ctx.timing_read()
+        ctx.timing(True)
+        ctx.step(a.steps)
         tm = ctx.timing_read()
         ctx.timing(False)
         state = ctx.get_state()
@@ -133,7 +201,8 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     sps = a.steps / (t1 - t0)
     pb, balg = pass_bytes(N, Jt, Kkept)
-    dom = max((k for k in tm if tm[k][1] > 0), key=lambda k: tm[k][0])
+    dom = a.traffic_pass if tm.get(a.traffic_pass, (0, 0))[1] > 0 else \
+        max((k for k in tm if tm[k][1] > 0), key=lambda k: tm[k][0])
     avg_ms = tm[dom][0] / tm[dom][1]
     achieved = pb[dom] / (avg_ms * 1e-3) / 1e9
     out = {
@@ -155,9 +224,14 @@ def main():
         "hbm_frac_step": balg * sps / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_step": balg,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_launch": pb[dom], "avg_launch_ms": avg_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["total"] if traffic and dom == a.traffic_pass else None,
+                     "alg_bytes_per_launch": pb[dom], "avg_launch_ms": avg_ms,
+                     "traffic_fetch_write": [traffic["fetch"], traffic["write"]] if traffic else None,
+                     "traffic_note": traffic_err or f"{a.traffic_pass}: rocprofv3 --pmc, counters scaled "
+                                                    f"{FETCH_SCALE:g}/{WRITE_SCALE:g} by a D2D copy"},
         "pass_ms_per_step": {k: tm[k][0] / a.steps for k in tm},
+        "kernel_ms_per_step": sum(tm[k][0] for k in tm) / a.steps,
         "finite": finite,
     }
     if not a.no_cpu_baseline:

@@ -13,7 +13,7 @@ namespace chns {
 
 
 // Column iFFT of a whole state field with a per-mode operator, optionally
-// truncated; writes the row-pair tiled [N][ldF] array.  grid: 1-D items.
+// truncated; writes the TJ-row tiled [N][ldF] array.  grid: 1-D items.
 template <int LOG2N, int OP, bool TRUNC>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_inv_full(const cd* __restrict__ S, cd* out,
                                                                   const cd* __restrict__ tw,
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_inv_full(const cd* __
     v[m] = z;
   }
   fft<LOG2N, CF::P, true, CF::C>(v, lds, tw, t);
-  col_store_rm<LOG2N, CF::P>(out, ldF, j, ok, t, v);
+  col_store_rm<LOG2N, CF::P, TJ>(out, ldF, j, ok, t, v);
 }
 
 // Paired row c2r of full half-spectra (all ky columns; real parts at ky = 0
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_c2r_full(const cd* __
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
   cd v[8];
-  const size_t ro = tix(x, 0, ldF);
+  const size_t ro = tix<TJ>(x, 0, ldF);
   row_load_pair<LOG2N, false>(X + ro, Y ? Y + ro : nullptr, H, 0.0, t, v);
   fft<LOG2N, RC::P, true, RC::R>(v, lds, tw, t);
 #pragma unroll

@@ -2,8 +2,8 @@
 //
 // Layouts in HBM (N = Nx = Ny, Ny2 = N/2+1):
 //   spectral state / saved N_n : column-major  S[j*N + i]   (j = ky index, i = kx index)
-//   row-pass inputs  (H*)      : row-pair tiled [N/2][ldJ][2], j < Jt (dealiased columns)
-//   row-pass outputs (Mh*, G*) : row-pair tiled [N/2][ldF][2], j < Ny2
+//   row-pass inputs  (H*)      : 2-row tiled [N/2][ldJ][2], j < Jt (dealiased columns)
+//   row-pass outputs (Mh*, G*) : 4-row tiled [N/4][ldF][4], j < Ny2
 // Column kernels (x direction) read the state contiguously and touch the
 // row-major arrays in chunks of C consecutive columns; row kernels (y
 // direction) are contiguous everywhere.
@@ -66,31 +66,40 @@ template <int LOG2N> struct RowCfg {
   static constexpr int NT = R * T;
 };
 
-// Row handled by lane-row r of this workgroup.  With R >= 2 the two rows of
-// a tiled pair share the workgroup; with R == 1 rows 2p and 2p+1 go to blocks
-// b and b+8, which sit on the same XCD (blocks are dealt to the 8 XCDs
-// round-robin) and start together, so the second half-line read hits L2.
-template <int LOG2N> __device__ __forceinline__ int row_of(int r) {
-  if constexpr (RowCfg<LOG2N>::R >= 2) {
-    return blockIdx.x * RowCfg<LOG2N>::R + r;
-  } else {
-    const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
-    return 2 * (((idx >> 1) << 3) + xcd) + (idx & 1);
-  }
+// Tiled layouts of the row-pass arrays: element (x, j) of a logical [N][ld]
+// array lives at ((x/TR)*ld + j)*TR + x%TR.  A column workgroup (2 columns at
+// N = 2048) then reads 32*TR contiguous bytes per row group: TF = 4 rows make
+// that a whole 128-byte line for the row-pass outputs (Mh*, G*), which the
+// column kernels read; TJ = 2 rows for the row-pass inputs (H*), which the
+// column kernels write and the row kernels read.
+constexpr int TJ = 2;
+constexpr int TF = 4;
+template <int TR>
+__device__ __forceinline__ size_t tix(int x, int j, int ld) {
+  return ((size_t)(x / TR) * ld + j) * TR + (x % TR);
 }
 
-// Row-pair tiled layout of the row-pass arrays: element (x, j) of a logical
-// [N][ld] array lives at ((x/2)*ld + j)*2 + x%2, so C consecutive columns of two
-// consecutive rows form one contiguous 32*C-byte chunk for the column kernels,
-// while a row kernel holding both rows of a pair still covers whole lines.
-__device__ __forceinline__ size_t tix(int x, int j, int ld) {
-  return (((size_t)(x >> 1) * ld + j) << 1) + (x & 1);
+// Row handled by lane-row r of this workgroup.  Workgroups holding fewer than
+// TF rows are grouped so that the TF rows of one output tile go to blocks
+// b, b+8, b+16, ... which sit on the same XCD (blocks are dealt to the 8 XCDs
+// round-robin) and start together: their partial-line writes of the outputs
+// merge in that L2, and the two rows of an input pair read one line.
+template <int LOG2N> __device__ __forceinline__ int row_of(int r) {
+  constexpr int R = RowCfg<LOG2N>::R;
+  if constexpr (R >= TF) {
+    return blockIdx.x * R + r;
+  } else {
+    constexpr int G = TF / R;   // workgroups per output tile
+    const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
+    return TF * (((idx / G) << 3) + xcd) + (idx % G) * R + r;
+  }
 }
 
 // Work item (tile, group) of a column kernel launched on a 1-D grid of
 // ceil(ntiles*ngroups/16)*16 blocks.  Items 2q and 2q+1 (neighbouring tiles)
 // go to blocks b and b+8, which sit on the same XCD and start together, so
-// the two halves of each 128-byte line of the tiled arrays meet in that L2.
+// the two halves of each 128-byte line of the 2-row tiled arrays they write
+// meet in that L2.
 // Returns false for padding blocks (the whole workgroup exits at once).
 __device__ __forceinline__ bool col_item(int ntiles, int ngroups, int* tile, int* group) {
   const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
@@ -102,18 +111,18 @@ __device__ __forceinline__ bool col_item(int ntiles, int ngroups, int* tile, int
 }
 
 // ---------------------------------------------------------------- column I/O
-template <int LOG2N, int P>
+template <int LOG2N, int P, int TR>
 __device__ __forceinline__ void col_load_rm(const cd* __restrict__ A, int ld, int j, bool ok, int t, cd* v) {
   constexpr int T = (1 << LOG2N) / P;
 #pragma unroll
-  for (int m = 0; m < P; ++m) v[m] = ok ? A[tix(t + m * T, j, ld)] : mk(0.0, 0.0);
+  for (int m = 0; m < P; ++m) v[m] = ok ? A[tix<TR>(t + m * T, j, ld)] : mk(0.0, 0.0);
 }
-template <int LOG2N, int P>
+template <int LOG2N, int P, int TR>
 __device__ __forceinline__ void col_store_rm(cd* __restrict__ A, int ld, int j, bool ok, int t, const cd* v) {
   constexpr int T = (1 << LOG2N) / P;
   if (ok) {
 #pragma unroll
-    for (int m = 0; m < P; ++m) A[tix(t + m * T, j, ld)] = v[m];
+    for (int m = 0; m < P; ++m) A[tix<TR>(t + m * T, j, ld)] = v[m];
   }
 }
 // masked load of a state column: modes with |i| > a are truncated (functions.c:75-78)
@@ -153,11 +162,11 @@ __device__ __forceinline__ void assemble_pair(const cd* X, const cd* Y, int st, 
   }
 }
 
-// Pair from the row-pair tiled arrays in global memory (row element k at +2k).
+// Pair from the TJ-row tiled arrays in global memory (row element k at +TJ*k).
 template <int LOG2N, bool YIK>
 __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd* __restrict__ Y, int Jt,
                                               double f, int t, cd* v) {
-  assemble_pair<LOG2N, YIK>(X, Y, 2, Jt, f, t, v);
+  assemble_pair<LOG2N, YIK>(X, Y, TJ, Jt, f, t, v);
 }
 
 // Streams the dealiased rows of the next field pair into LDS while the
@@ -177,9 +186,9 @@ struct RowStager {
     for (int base = wave * 64; base < nslot; base += T) {
       const int s = base + lane;
       if (s < nslot) {
-        const cd* src = X + 2 * s;
+        const cd* src = X + TJ * s;
         if constexpr (HASY) {
-          if (s >= Jt) src = Y + 2 * (s - Jt);
+          if (s >= Jt) src = Y + TJ * (s - Jt);
         }
         __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(stg + base), 16, 0, 0);
       }
@@ -232,7 +241,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Bufs B, KParam
   cd v[CF::P];
   col_load_masked<LOG2N, CF::P>(S, j, ok ? B.amask[j] : -1, t, v);
   fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-  col_store_rm<LOG2N, CF::P>(H, P.ldJ, j, ok, t, v);
+  col_store_rm<LOG2N, CF::P, TJ>(H, P.ldJ, j, ok, t, v);
 }
 
 // ============================================================ pass B
@@ -246,8 +255,8 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
   cd v[8];
-  const cd* h1 = B.Hc1 + tix(x, 0, P.ldJ);
-  const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix(x, 0, P.ldJ) : nullptr;
+  const cd* h1 = B.Hc1 + tix<TJ>(x, 0, P.ldJ);
+  const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix<TJ>(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
   fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
@@ -273,11 +282,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
     }
   }
   fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
-  cd* o1 = B.Mh1 + tix(x, 0, P.ldF);
-  cd* o2 = B.Mh2 + tix(x, 0, P.ldF);
+  cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
+  cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
-    o1[2 * n] = a;
-    if (MODEL == MODEL_TERNARY) o2[2 * n] = b;
+    o1[TF * n] = a;
+    if (MODEL == MODEL_TERNARY) o2[TF * n] = b;
   });
 }
 
@@ -303,7 +312,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
   if (g <= 1) {
     // mu_i = T(M_i + 0.75 g_i eps k^2 c_i)  (surface_tension.c:116-128)
     // binary: mu = T(1.5 sigma eps k^2 c + 24 sigma/eps M)  (surface_tension.c:139-142,166)
-    col_load_rm<LOG2N, CF::P>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
+    col_load_rm<LOG2N, CF::P, TF>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
     fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
     col_load_masked<LOG2N, CF::P>(g == 0 ? B.C1 : B.C2, j, a, t, s);
     const double mc = g == 0 ? P.mc1 : P.mc2;
@@ -319,7 +328,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
       }
     }
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
     col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
@@ -331,35 +340,35 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
       v[m] = ctimes_ik(s[m], ky);
     }
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(B.Hu, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(B.Hv, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
     col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(B.Hwx, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], ky);
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(B.Hwy, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
     col_load_masked<LOG2N, CF::P>(B.C1, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(B.Hdx1, P.ldJ, j, ok, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hdx1, P.ldJ, j, ok, t, v);
     if constexpr (MODEL == MODEL_TERNARY) {
       col_load_masked<LOG2N, CF::P>(B.C2, j, a, t, s);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
       fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-      col_store_rm<LOG2N, CF::P>(B.Hdx2, P.ldJ, j, ok, t, v);
+      col_store_rm<LOG2N, CF::P, TJ>(B.Hdx2, P.ldJ, j, ok, t, v);
     }
   }
 }
@@ -374,8 +383,8 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
-  const size_t ro = tix(x, 0, P.ldJ);   // row base in the tiled layout; element k at +2k
-  const size_t fo = tix(x, 0, P.ldF);
+  const size_t ro = tix<TJ>(x, 0, P.ldJ);   // row bases in the tiled layouts: element k at +TJ*k
+  const size_t fo = tix<TF>(x, 0, P.ldF);   // ... and at +TF*k
   const double sc = P.scale;
   cd v[8];
   if constexpr (MODEL == MODEL_NS) {
@@ -391,28 +400,29 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
     fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
     cd* o = B.G1 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[2 * n] = mk(-A.x, -A.y); });
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
   } else {
-    // The order keeps few real fields live: first the surface-tension force
-    // F = sum_i A_i grad c_i (surface_tension.c:138-220), transformed at once
-    // (F(Fx) kept in registers, F(Fy) stored), then the advection products.
+    // Order: mu -> A_i, then velocity, then grad c_i (surface-tension force
+    // F = sum_i A_i grad c_i, surface_tension.c:138-220, and advection u.grad c_i,
+    // nonlin.c:266-348, accumulate as each pair lands), then grad w for u.grad w
+    // (nonlin.c:56-62).  (Fy, u.grad c1) are transformed as soon as they are
+    // complete; -u.grad w and Fx share the next forward transform, so
+    // W1 = -F(u.grad w) - i ky F(Fx) (curl.c:26, ky part) comes out of its split.
     constexpr bool TER = MODEL == MODEL_TERNARY;
     // One row per workgroup (N = 2048): each field pair is staged in LDS
     // while the previous pair is transformed.
     constexpr bool STG = RC::R == 1;
     __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
     const RowStager<LOG2N> st{stg_all, P.Jt};
-    if constexpr (STG) {
-      st.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
-      st.wait();
-    }
-    double fx[8], fy[8], d1x[8], d1y[8], d2x[8], d2y[8];
+    double u[8], w[8], fx[8], fy[8], ad1[8], ad2[8];
     {
       double A1[8], A2[8];
       if constexpr (STG) {
+        st.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
+        st.wait();
         st.template assemble<false, TER>(P.f, t, v);
         lds_sync();
-        st.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+        st.template issue<true>(B.Hu + ro, B.Hv + ro);
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
@@ -429,66 +439,63 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
       }
       if constexpr (STG) {
         st.wait();
+        st.template assemble<false, true>(P.f, t, v);
+        lds_sync();
+        st.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+      } else {
+        row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+      }
+      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        u[m] = v[m].x * sc;
+        w[m] = v[m].y * sc;
+      }
+      if constexpr (STG) {
+        st.wait();
         st.template assemble<true, true>(P.f, t, v);
         lds_sync();
         if (TER) st.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
-        else st.template issue<true>(B.Hu + ro, B.Hv + ro);
+        else st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
       fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        d1x[m] = v[m].x * sc;
-        d1y[m] = v[m].y * sc;
-        fx[m] = d1x[m] * A1[m];
-        fy[m] = d1y[m] * A1[m];
+        const double dx = v[m].x * sc, dy = v[m].y * sc;
+        fx[m] = dx * A1[m];
+        fy[m] = dy * A1[m];
+        ad1[m] = dx * u[m] + dy * w[m];
       }
       if constexpr (TER) {
         if constexpr (STG) {
           st.wait();
           st.template assemble<true, true>(P.f, t, v);
           lds_sync();
-          st.template issue<true>(B.Hu + ro, B.Hv + ro);
+          st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
         fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
-          d2x[m] = v[m].x * sc;
-          d2y[m] = v[m].y * sc;
-          fx[m] += d2x[m] * A2[m];
-          fy[m] += d2y[m] * A2[m];
+          const double dx = v[m].x * sc, dy = v[m].y * sc;
+          fx[m] += dx * A2[m];
+          fy[m] += dy * A2[m];
+          ad2[m] = dx * u[m] + dy * w[m];
         }
       }
     }
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(fx[m], fy[m]);
+    for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
     fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
-    // F(Fx) waits in LDS (not registers) until F(u.grad w) is ready
-    __shared__ cd fxs[RC::R * 5 * T];
-    cd* fxh = fxs + (size_t)t * RC::R + r;
     cd* o2 = B.G2 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) { fxh[m * T * RC::R] = A; o2[2 * n] = Bv; });
-    // advection: u.grad c_i (nonlin.c:266-348) and u.grad w (nonlin.c:56-62)
-    double u[8], w[8], ad1[8], ad2[8];
-    if constexpr (STG) {
-      st.wait();
-      st.template assemble<false, true>(P.f, t, v);
-      lds_sync();
-      st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
-    } else {
-      row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    }
-    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      u[m] = v[m].x * sc;
-      w[m] = v[m].y * sc;
-      ad1[m] = d1x[m] * u[m] + d1y[m] * w[m];
-      if (TER) ad2[m] = d2x[m] * u[m] + d2y[m] * w[m];
-    }
+    cd* o3 = B.G3 + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+      o2[TF * n] = A;
+      o3[TF * n] = Bv;
+    });
     if constexpr (STG) {
       st.wait();
       st.template assemble<false, true>(P.f, t, v);
@@ -497,22 +504,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
     }
     fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), ad1[m]);
+    for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
     fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
-    // W1 = -F(u.grad w) - i ky F(Fx): the -ky part of curl (curl.c:26) applied row-side
     cd* o1 = B.G1 + fo;
-    cd* o3 = B.G3 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int m, int n, cd A, cd Bv) {
-      const cd iky = ctimes_ik(fxh[m * T * RC::R], P.f * (double)n);
-      o1[2 * n] = mk(-A.x - iky.x, -A.y - iky.y);
-      o3[2 * n] = Bv;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+      const cd iky = ctimes_ik(Bv, P.f * (double)n);
+      o1[TF * n] = mk(A.x - iky.x, A.y - iky.y);
     });
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
       fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
       cd* o4 = B.G4 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[2 * n] = A; });
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[TF * n] = A; });
     }
   }
 }
@@ -534,12 +538,12 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
   const double ky = P.f * (double)j;
   cd v[CF::P], s[CF::P];
   const cd* Gm = g == 0 ? B.G1 : (g == 1 ? B.G3 : B.G4);
-  col_load_rm<LOG2N, CF::P>(Gm, P.ldF, j, ok, t, v);
+  col_load_rm<LOG2N, CF::P, TF>(Gm, P.ldF, j, ok, t, v);
   fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
   if (g == 0) {
     if constexpr (MODEL != MODEL_NS) {
       // + i kx F(Fy): the curl of the surface-tension force (curl.c:26)
-      col_load_rm<LOG2N, CF::P>(B.G2, P.ldF, j, ok, t, s);
+      col_load_rm<LOG2N, CF::P, TF>(B.G2, P.ldF, j, ok, t, s);
       fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
@@ -547,7 +551,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
     if (P.f_k > 0 && j == P.f_k && t == 0) v[0].y += P.force_im;   // Kolmogorov forcing at (0, f_k)
   } else {
     // N_c = -F(u.grad c) - (lam/g) k^2 M  (nonlin.c:369-370); binary: - lam*(24 s/e k^2 M) (:240,251)
-    col_load_rm<LOG2N, CF::P>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
+    col_load_rm<LOG2N, CF::P, TF>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
     fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
     const double lg = g == 1 ? P.lg1 : P.lg2;
 #pragma unroll
@@ -593,7 +597,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
+    col_store_rm<LOG2N, CF::P, TJ>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }
 
@@ -617,11 +621,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
   fft<LOG2N, RC::P, false, RC::R>(v, lds, tw, t);
-  cd* pa = oa + tix(x, 0, ldF);
-  cd* pb = ob ? ob + tix(x, 0, ldF) : nullptr;
+  cd* pa = oa + tix<TF>(x, 0, ldF);
+  cd* pb = ob ? ob + tix<TF>(x, 0, ldF) : nullptr;
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
-    pa[2 * n] = A;
-    if (pb) pb[2 * n] = Bv;
+    pa[TF * n] = A;
+    if (pb) pb[TF * n] = Bv;
   });
 }
 
@@ -640,7 +644,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* _
   const bool ok = j < H;
   cd* lds = lds_all + c;
   cd v[CF::P];
-  col_load_rm<LOG2N, CF::P>(in, ldF, j, ok, t, v);
+  col_load_rm<LOG2N, CF::P, TF>(in, ldF, j, ok, t, v);
   fft<LOG2N, CF::P, false, CF::C>(v, lds, tw, t);
   if (ok) {
     const int a = TRUNC ? amask[j] : N;
