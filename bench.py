@@ -45,8 +45,10 @@ WRITE_SCALE = 1.0
 def kernel_tag(pass_name: str, N: int) -> str:
     """Substring of the demangled kernel name of a pass (ternary model)."""
     L = int(round(math.log2(N)))
-    return {"colA": f"k_colA<{L}>", "rowB": f"k_rowB<{L}, 2>", "colC": f"k_colC<{L}, 2>",
-            "rowD": f"k_rowD<{L}, 2>", "colE1": f"k_colE<{L}, 2, 1>", "colE2": f"k_colE<{L}, 2, 2>"}[pass_name]
+    inst = "chns::Inst>"   # single-instance launches (ensembles use chns::InstBatch)
+    return {"colA": f"k_colA<{L}, {inst}", "rowB": f"k_rowB<{L}, 2, {inst}", "colC": f"k_colC<{L}, 2, {inst}",
+            "rowD": f"k_rowD<{L}, 2, {inst}", "colE1": f"k_colE<{L}, 2, 1, {inst}",
+            "colE2": f"k_colE<{L}, 2, 2, {inst}"}[pass_name]
 
 
 def measure_traffic(N: int, tag: str, timeout: int = 120):
@@ -128,6 +130,93 @@ def cpu_baseline(N, kw, seconds=10.0, max_steps=50):
                       f"1 thread), {dt:.1f} s"}
 
 
+def ensemble_members():
+    """The batched-ensemble workload: 32 x 256^2 + 16 x 512^2 + 4 x 1024^2 instances;
+    within each size even b is binary (c = 1/2 + U[-0.05, 0.05]) and odd b ternary
+    (c1, c2 = 1/3 + U[-0.05, 0.05]); sigma (each sigma_ij) ~ U[0.05, 0.15] from
+    SplitMix64 seeded 1000 + b, the noise seeded 2000 + b; epsilon = 1.5 dx, lambda = 1e-3,
+    nu = 5e-3, dt = 5e-4 (256/N)^3, L = 2 pi, omega_0 = 0."""
+
+    from chns_amd import ic
+
+    out = []
+    b = 0
+    for N, count in ((256, 32), (512, 16), (1024, 4)):
+        for k in range(count):
+            model = "binary" if k % 2 == 0 else "ternary"
+            L = 2.0 * math.pi
+            sig = 0.05 + 0.1 * ic._splitmix64(1000 + b, 3)
+            # dt = 5e-4 at 256^2; the explicit k^2 muF rate grows like N^3 at epsilon = 1.5 dx
+            # and 5e-4 diverges at 512^2 and 1024^2, so dt shrinks as (256/N)^3
+            kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=5e-4 * (256.0 / N) ** 3, lam=1e-3, epsilon=1.5 * L / N,
+                      sigma=float(sig[0]), sigma_12=float(sig[0]), sigma_13=float(sig[1]), sigma_23=float(sig[2]))
+            nf = 1 if model == "binary" else 2
+            fields = ic.noise(N, N, 2000 + b, 0.5 if model == "binary" else 1.0 / 3.0, 0.05, nf)
+            out.append((model, kw, fields))
+            b += 1
+    return out
+
+
+def ensemble_bench(steps: int, warmup: int):
+    """Speedup of one graph-launched batched ensemble over the same instances run
+    one after another, each alone on the GPU (sum of their timed steps)."""
+    import numpy as np
+
+    import chns_amd as C
+
+    members = ensemble_members()
+    ps = [C.params(m, **kw) for m, kw, _ in members]
+    seq = 0.0
+    for p, (m, kw, f) in zip(ps, members):
+        with C.Context(p) as ctx:
+            ctx.set_real_state(None, *f)
+            ctx.step(warmup)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step(steps)
+            ctx.sync()
+            seq += time.perf_counter() - t0
+    res = {}
+    with C.Ensemble(ps) as ens:
+        for b, (m, kw, f) in enumerate(members):
+            ens[b].set_real_state(None, *f)
+        for tuned in (False, True):
+            splits = ens.tune(reps=3) if tuned else None
+            ens.step(warmup)
+            ens.sync()
+            t0 = time.perf_counter()
+            ens.step(steps)
+            ens.sync()
+            res[tuned] = (time.perf_counter() - t0, splits)
+        finite = all(bool(np.all(np.isfinite(x))) for b in range(len(ens)) for x in ens[b].get_state())
+        groups = ens.groups()
+    best = min(res, key=lambda k: res[k][0])
+    batch = res[best][0]
+    pts = sum(kw["Nx"] * kw["Ny"] for _, kw, _ in members)
+    return {
+        "metric": "batched ensemble speedup over sequential single-instance runs",
+        "value": seq / batch,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": batch * 1e3 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 noise, seeds 1000+b / 2000+b)",
+        "config": {"workload": "ensemble 32x256^2 + 16x512^2 + 4x1024^2, binary/ternary alternating, "
+                               "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3, one HIP graph replay per step",
+                   "instances": len(members), "groups": groups, "parallelism": "single GPU"},
+        "sequential_s": seq,
+        "batch_s": {"untuned": res[False][0], "tuned": res[True][0]},
+        "tuned_splits": res[True][1],
+        "grid_pt_steps_per_s": pts * steps / batch,
+        "finite": finite,
+    }
+
+
 def copy_probe(nbytes: int, reps: int = 4):
     """Device-to-device copies of a known size (hipMemcpy) for calibrating the
     FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel."""
@@ -156,9 +245,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--traffic-pass", default="rowD", help="pass whose HBM traffic is counted")
+    ap.add_argument("--ensemble", action="store_true", help="measure the batched-ensemble workload instead")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
+        return
+    if a.ensemble:
+        if a.gpus != 1:
+            print("bench.py: the ensemble runs on one GPU", file=sys.stderr)
+            sys.exit(2)
+        print(json.dumps(ensemble_bench(a.steps if a.steps != 200 else 500, a.warmup)))
         return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "

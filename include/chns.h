@@ -124,6 +124,37 @@ int chns_timing_enable(chns_ctx *ctx, int enable);
  * counts per pass kind since the last read, and resets them. */
 int chns_timing_read(chns_ctx *ctx, double *total_ms, int *launches);
 
+/* ---- ensembles (no reference counterpart: the reference runs one grid per
+ * MPI job; these run many independent instances on one GPU) -------------- */
+
+typedef struct chns_ensemble chns_ensemble;
+
+/* n independent instances, instance b with parameters p[b] (any mix of grid
+ * sizes and models).  Instances of equal size and model form a group whose
+ * passes run as one batched launch each; groups run concurrently. */
+int chns_ensemble_create(chns_ensemble **out, const chns_params *p, int n, int device);
+void chns_ensemble_destroy(chns_ensemble *e);
+const char *chns_ensemble_last_error(const chns_ensemble *e);
+int chns_ensemble_size(const chns_ensemble *e);
+/* The context of instance b (NULL if out of range): state transfer,
+ * diagnostics and single-instance steps through the ordinary calls above.
+ * Its work runs on the ensemble's stream, ordered with chns_ensemble_step.
+ * Owned by the ensemble: chns_destroy on it does nothing. */
+chns_ctx *chns_ensemble_instance(chns_ensemble *e, int b);
+/* nsteps ETD2RK steps of every instance (each instance needs a state).  One
+ * step of the whole ensemble is captured into a HIP graph on first use and
+ * replayed per step.  Returns when queued; chns_ensemble_sync waits. */
+int chns_ensemble_step(chns_ensemble *e, int nsteps);
+int chns_ensemble_sync(chns_ensemble *e);
+/* Autotune the launch split of every group (1, 2 or 4 concurrent slices),
+ * timing `reps` replayed steps per candidate, largest groups first; the
+ * instances' states are restored afterwards.  splits (may be NULL) receives
+ * the choice per group, in group order. */
+int chns_ensemble_tune(chns_ensemble *e, int reps, int *splits);
+/* Groups in order: log2 N, model and instance count (arrays may be NULL);
+ * returns the number of groups. */
+int chns_ensemble_groups(const chns_ensemble *e, int *log2n, int *model, int *count);
+
 /* ---- host-only helpers (no device needed) ---------------------------- */
 
 /* ETD2RK coefficients per mode, IDXC order [Nx][Ny2] each, in the exact

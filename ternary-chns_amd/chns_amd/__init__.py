@@ -31,7 +31,9 @@ EXPORTS = (
     "chns_get_state", "chns_set_real_state", "chns_step", "chns_sync", "chns_truncate_state",
     "chns_etd_tables", "chns_dealias_extent", "chns_timing_enable", "chns_timing_read",
     "chns_free_energy", "chns_kinetic_energy", "chns_perimeter", "chns_spectrum", "chns_real_field",
-    "chns_diag_sizes",
+    "chns_diag_sizes", "chns_ensemble_create", "chns_ensemble_destroy", "chns_ensemble_last_error",
+    "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
+    "chns_ensemble_tune", "chns_ensemble_groups",
 )
 PASS_NAMES = ("colA", "rowB", "colC", "rowD", "colE1", "colE2")
 
@@ -84,6 +86,18 @@ def load_library(path: str = LIB_PATH):
     lib.chns_dealias_extent.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
     lib.chns_timing_enable.argtypes = [V, I]
     lib.chns_timing_read.argtypes = [V, D, ctypes.POINTER(ctypes.c_int)]
+    lib.chns_ensemble_create.argtypes = [ctypes.POINTER(V), P, I, I]
+    lib.chns_ensemble_destroy.argtypes = [V]
+    lib.chns_ensemble_destroy.restype = None
+    lib.chns_ensemble_last_error.argtypes = [V]
+    lib.chns_ensemble_last_error.restype = ctypes.c_char_p
+    lib.chns_ensemble_size.argtypes = [V]
+    lib.chns_ensemble_instance.argtypes = [V, I]
+    lib.chns_ensemble_instance.restype = V
+    lib.chns_ensemble_step.argtypes = [V, I]
+    lib.chns_ensemble_sync.argtypes = [V]
+    lib.chns_ensemble_tune.argtypes = [V, I, ctypes.POINTER(I)]
+    lib.chns_ensemble_groups.argtypes = [V, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
     lib.chns_free_energy.argtypes = [V, D, D]
     lib.chns_kinetic_energy.argtypes = [V, D, D, D]
     lib.chns_perimeter.argtypes = [V, D]
@@ -126,12 +140,16 @@ def _nphase(model):
 class Context:
     """A device-resident CHNS state (one ``chns_ctx``)."""
 
-    def __init__(self, p: CParams, device: int = 0):
+    def __init__(self, p: CParams, device: int = 0, _handle=None):
         self.lib = load_library()
         self.p = p
         self.Nx, self.Ny = p.Nx, p.Ny
         self.Ny2 = p.Ny // 2 + 1
         self.nph = _nphase(p.model)
+        self._owned = _handle is None
+        if _handle is not None:          # an ensemble's instance (not owned)
+            self.h = _handle
+            return
         h = ctypes.c_void_p()
         rc = self.lib.chns_create(ctypes.byref(h), ctypes.byref(p), device)
         if rc != CHNS_OK:
@@ -145,7 +163,8 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.chns_destroy(self.h)
+            if self._owned:
+                self.lib.chns_destroy(self.h)
             self.h = None
 
     __del__ = close
@@ -251,6 +270,68 @@ class Context:
         self._check(self.lib.chns_timing_read(self.h, _dp(ms), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int))),
                     "chns_timing_read")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(PASS_NAMES)}
+
+
+class Ensemble:
+    """Many independent instances on one GPU (``chns_ensemble``): batched
+    launches per (size, model) group, one captured HIP graph per step."""
+
+    def __init__(self, plist, device: int = 0):
+        self.lib = load_library()
+        self.plist = list(plist)
+        arr = (CParams * len(self.plist))(*self.plist)
+        h = ctypes.c_void_p()
+        rc = self.lib.chns_ensemble_create(ctypes.byref(h), arr, len(self.plist), device)
+        if rc != CHNS_OK:
+            raise ChnsError(f"chns_ensemble_create failed ({rc})")
+        self.h = h
+        self.instances = [Context(p, _handle=ctypes.c_void_p(self.lib.chns_ensemble_instance(h, b)))
+                          for b, p in enumerate(self.plist)]
+
+    def _check(self, rc, what):
+        if rc != CHNS_OK:
+            raise ChnsError(f"{what} failed ({rc}): {self.lib.chns_ensemble_last_error(self.h).decode()}")
+
+    def __len__(self):
+        return len(self.instances)
+
+    def __getitem__(self, b) -> Context:
+        return self.instances[b]
+
+    def step(self, nsteps: int = 1):
+        self._check(self.lib.chns_ensemble_step(self.h, int(nsteps)), "chns_ensemble_step")
+
+    def sync(self):
+        self._check(self.lib.chns_ensemble_sync(self.h), "chns_ensemble_sync")
+
+    def groups(self):
+        """[(log2 N, model, count)] in group order."""
+        n = self.lib.chns_ensemble_groups(self.h, None, None, None)
+        a, b, c = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_int * n)()
+        self.lib.chns_ensemble_groups(self.h, a, b, c)
+        return [(a[k], b[k], c[k]) for k in range(n)]
+
+    def tune(self, reps: int = 5):
+        """Autotune the launch split per group; returns the chosen splits."""
+        n = len(self.groups())
+        out = (ctypes.c_int * n)()
+        self._check(self.lib.chns_ensemble_tune(self.h, int(reps), out), "chns_ensemble_tune")
+        return list(out)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for c in self.instances:
+                c.h = None
+            self.lib.chns_ensemble_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 # ---------------------------------------------------------------- reference-named mirrors

@@ -131,14 +131,18 @@ extern "C" const char* chns_last_error(const chns_ctx* ctx) {
 }
 
 extern "C" void chns_destroy(chns_ctx* c) {
-  if (!c) return;
+  if (!c || c->in_ensemble) return;   // an ensemble's instances go with the ensemble
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* q : c->allocs) (void)hipFree(q);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
 extern "C" int chns_create(chns_ctx** out, const chns_params* p, int device) {
+  return chns_create_on(out, p, device, nullptr);
+}
+
+int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t shared) {
   if (!out) return CHNS_E_ARG;
   *out = nullptr;
   if (const char* why = chns_host::check_params(p)) {
@@ -158,8 +162,13 @@ extern "C" int chns_create(chns_ctx** out, const chns_params* p, int device) {
   }
   int rc = 0;
   auto bail = [&](int r) { int code = r; std::fprintf(stderr, "chns_create: %s\n", c->err.c_str()); chns_destroy(c); return code; };
-  e = hipStreamCreate(&c->stream);
-  if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
+  if (shared) {
+    c->stream = shared;
+    c->own_stream = false;
+  } else {
+    e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) return bail(hip_fail(c, e, "hipStreamCreate"));
+  }
 
   const int N = c->d.N, H = c->d.H;
   c->amask.assign(H, -1);

@@ -1,0 +1,28 @@
+// Batched pass launchers for ensembles (gfx950): nb instances of one grid size
+// and model per launch, instance b's buffers and parameters at B[b], P[b]
+// (device arrays), grid.z = b.
+#include <hip/hip_runtime.h>
+
+#include "chns_launch.hpp"
+#include "chns_passes.hpp"
+
+namespace chns {
+
+int launch_batch_colA(int log2n, const Bufs* B, const KParams* P, int Jt, int nph, int nb, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::colA<L>(InstBatch{B, P}, Jt, nph, nb, s)));
+}
+int launch_batch_rowB(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::rowB<L>(model, InstBatch{B, P}, nb, s)));
+}
+int launch_batch_colC(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::colC<L>(model, InstBatch{B, P}, Jt, nb, s)));
+}
+int launch_batch_rowD(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::rowD<L>(model, InstBatch{B, P}, nb, s)));
+}
+int launch_batch_colE(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb,
+                      hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::colE<L>(model, stage, InstBatch{B, P}, nf, nb, s)));
+}
+
+}  // namespace chns

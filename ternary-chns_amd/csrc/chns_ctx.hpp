@@ -14,6 +14,8 @@ struct chns_ctx {
   chns_host::Derived d{};
   int dev = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;                    // false: an ensemble's stream
+  bool in_ensemble = false;                  // owned by a chns_ensemble
   int Jt = 0, ldJ = 0, ldF = 0, nph = 0;
   chns::KParams kp{};
   chns::Bufs b{};
@@ -36,3 +38,5 @@ struct chns_ctx {
 int chns_fail(chns_ctx* c, int code, const std::string& msg);
 int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what);
 int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes);
+// chns_create on a caller-owned stream (nullptr: the context creates its own)
+int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t shared);

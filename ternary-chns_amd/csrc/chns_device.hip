@@ -1,52 +1,16 @@
 // Kernel instantiations and launch wrappers (gfx950).
 #include <hip/hip_runtime.h>
 
-#include "chns_kernels.hpp"
+#include "chns_passes.hpp"
 #include "chns_diag.hpp"
 #include "chns_launch.hpp"
 
 namespace chns {
 
 namespace {
-template <int L, int PP = 8> constexpr int ceil_div(int a) { return (a + ColCfg<L, PP>::C - 1) / ColCfg<L, PP>::C; }
+using passes::ceil_div;
+using passes::round16;
 
-int round16(int n) { return (n + 15) / 16 * 16; }
-int groupsC(int model) { return model == MODEL_TERNARY ? 5 : (model == MODEL_BINARY ? 4 : 2); }
-
-template <int L> int colA(const Bufs& B, const KParams& P, int nph, hipStream_t s) {
-  k_colA<L><<<dim3(ceil_div<L, 16>(P.Jt), nph), ColCfg<L, 16>::NT, 0, s>>>(B, P);
-  return (int)hipGetLastError();
-}
-template <int L> int rowB(int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  const dim3 g(P.N / RowCfg<L>::R);
-  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY><<<g, RowCfg<L>::NT, 0, s>>>(B, P);
-  else k_rowB<L, MODEL_BINARY><<<g, RowCfg<L>::NT, 0, s>>>(B, P);
-  return (int)hipGetLastError();
-}
-template <int L> int colC(int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  const dim3 g(round16(ceil_div<L>(P.Jt) * groupsC(model)));
-  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  else k_colC<L, MODEL_NS><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  return (int)hipGetLastError();
-}
-template <int L> int rowD(int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  const dim3 g(P.N / RowCfg<L>::R);
-  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY><<<g, RowCfg<L>::NT, 0, s>>>(B, P);
-  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY><<<g, RowCfg<L>::NT, 0, s>>>(B, P);
-  else k_rowD<L, MODEL_NS><<<g, RowCfg<L>::NT, 0, s>>>(B, P);
-  return (int)hipGetLastError();
-}
-template <int L, int ST> int colE_st(int model, const Bufs& B, const KParams& P, int nf, hipStream_t s) {
-  const dim3 g(round16(ceil_div<L>(P.N / 2 + 1) * nf));
-  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  else k_colE<L, MODEL_NS, ST><<<g, ColCfg<L>::NT, 0, s>>>(B, P);
-  return (int)hipGetLastError();
-}
-template <int L> int colE(int model, int stage, const Bufs& B, const KParams& P, int nf, hipStream_t s) {
-  return stage == 1 ? colE_st<L, 1>(model, B, P, nf, s) : colE_st<L, 2>(model, B, P, nf, s);
-}
 template <int L>
 int row_r2c(const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int ldF, hipStream_t s) {
   k_row_r2c<L><<<(1 << L) / RowCfg<L>::R, RowCfg<L>::NT, 0, s>>>(a, b, oa, ob, tw, ldF);
@@ -61,33 +25,20 @@ int col_fwd(bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, i
 }
 }  // namespace
 
-#define CHNS_DISPATCH(log2n, CALL)          \
-  switch (log2n) {                          \
-    case 4: { constexpr int L = 4; return CALL; }   \
-    case 5: { constexpr int L = 5; return CALL; }   \
-    case 6: { constexpr int L = 6; return CALL; }   \
-    case 7: { constexpr int L = 7; return CALL; }   \
-    case 8: { constexpr int L = 8; return CALL; }   \
-    case 9: { constexpr int L = 9; return CALL; }   \
-    case 10: { constexpr int L = 10; return CALL; } \
-    case 11: { constexpr int L = 11; return CALL; } \
-    default: return (int)hipErrorInvalidValue;      \
-  }
-
 int launch_colA(int log2n, const Bufs& B, const KParams& P, int nph, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (colA<L>(B, P, nph, s)));
+  CHNS_DISPATCH(log2n, (passes::colA<L>(Inst{B, P}, P.Jt, nph, 1, s)));
 }
 int launch_rowB(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (rowB<L>(model, B, P, s)));
+  CHNS_DISPATCH(log2n, (passes::rowB<L>(model, Inst{B, P}, 1, s)));
 }
 int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (colC<L>(model, B, P, s)));
+  CHNS_DISPATCH(log2n, (passes::colC<L>(model, Inst{B, P}, P.Jt, 1, s)));
 }
 int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (rowD<L>(model, B, P, s)));
+  CHNS_DISPATCH(log2n, (passes::rowD<L>(model, Inst{B, P}, 1, s)));
 }
 int launch_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nf, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (colE<L>(model, stage, B, P, nf, s)));
+  CHNS_DISPATCH(log2n, (passes::colE<L>(model, stage, Inst{B, P}, nf, 1, s)));
 }
 int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int ldF,
                    hipStream_t s) {

@@ -95,6 +95,22 @@ template <int LOG2N> __device__ __forceinline__ int row_of(int r) {
   }
 }
 
+// Kernel arguments of the pass kernels: one instance passed by value, or a
+// batch of same-size instances whose buffers and parameters sit in device
+// arrays indexed by blockIdx.z (ensembles).  The pass bodies only see B, P.
+struct Inst {
+  Bufs B;
+  KParams P;
+};
+struct InstBatch {
+  const Bufs* __restrict__ B;
+  const KParams* __restrict__ P;
+};
+__device__ __forceinline__ const Bufs& inst_bufs(const Inst& a) { return a.B; }
+__device__ __forceinline__ const KParams& inst_kp(const Inst& a) { return a.P; }
+__device__ __forceinline__ const Bufs& inst_bufs(const InstBatch& a) { return a.B[blockIdx.z]; }
+__device__ __forceinline__ const KParams& inst_kp(const InstBatch& a) { return a.P[blockIdx.z]; }
+
 // Work item (tile, group) of a column kernel launched on a 1-D grid of
 // ceil(ntiles*ngroups/16)*16 blocks.  Items 2q and 2q+1 (neighbouring tiles)
 // go to blocks b and b+8, which sit on the same XCD and start together, so
@@ -227,8 +243,10 @@ __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
 
 // ============================================================ pass A
 // grid: (ceil(Jt/C), nphase)
-template <int LOG2N>
-__global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Bufs B, KParams P) {
+template <int LOG2N, class Args>
+__global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
   using CF = ColCfg<LOG2N, 16>;
   constexpr int N = CF::N;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
@@ -246,8 +264,10 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Bufs B, KParam
 
 // ============================================================ pass B
 // grid: N / R
-template <int LOG2N, int MODEL>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
+template <int LOG2N, int MODEL, class Args>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
@@ -292,8 +312,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Bufs B, KParams P) {
 
 // ============================================================ pass C
 // grid: 1-D over (tile, group) items (col_item): ternary 5 groups, binary 4, NS 2.
-template <int LOG2N, int MODEL>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
+template <int LOG2N, int MODEL, class Args>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
@@ -375,8 +397,10 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Bufs B, KParams P) {
 
 // ============================================================ pass D
 // grid: N / R
-template <int LOG2N, int MODEL>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P) {
+template <int LOG2N, int MODEL, class Args>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
@@ -523,8 +547,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Bufs B, KParams P
 
 // ============================================================ pass E
 // grid: 1-D over (tile, field) items (col_item): omega, then phase field(s).
-template <int LOG2N, int MODEL, int STAGE>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Bufs B, KParams P) {
+template <int LOG2N, int MODEL, int STAGE, class Args>
+__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
@@ -658,7 +684,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* _
 
 // In-place truncation of a column-major spectral field (spectrum.c:24-26).
 // grid: ceil(Ny2*N / 256)
-__global__ void k_truncate(cd* S, const int* __restrict__ amask, int N, int H) {
+static __global__ void k_truncate(cd* S, const int* __restrict__ amask, int N, int H) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)N * H) return;
   const int j = (int)(e / N), i = (int)(e % N);

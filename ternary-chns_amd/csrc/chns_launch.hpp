@@ -1,5 +1,5 @@
-// Launch wrappers between the host API (chns_api.cpp) and the kernels
-// (chns_device.hip).  Each returns the hipError_t of the launch as int.
+// Launch wrappers between the host API (chns_api.cpp, chns_ensemble.cpp) and
+// the kernels (chns_device.hip, chns_batch.hip).  Each returns the hipError_t of the launch as int.
 #pragma once
 #include <hip/hip_runtime_api.h>
 
@@ -19,6 +19,15 @@ int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, c
                    hipStream_t s);
 int launch_truncate(cd* S, const int* amask, int N, hipStream_t s);
 int col_tile(int log2n);   // columns per column-kernel workgroup
+
+// batched passes (chns_batch.hip): nb instances of one size and model, device
+// arrays B[nb], P[nb]
+int launch_batch_colA(int log2n, const Bufs* B, const KParams* P, int Jt, int nph, int nb, hipStream_t s);
+int launch_batch_rowB(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
+int launch_batch_colC(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, hipStream_t s);
+int launch_batch_rowD(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
+int launch_batch_colE(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb,
+                      hipStream_t s);
 
 // diagnostics (chns_diag.hpp)
 struct RedArgs;

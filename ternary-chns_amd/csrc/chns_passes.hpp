@@ -1,0 +1,67 @@
+// Launch shapes of the five pass kernels, shared by the single-instance
+// launchers (chns_device.hip) and the batched ensemble launchers
+// (chns_batch.hip).  `a` is an Inst (nb = 1) or an InstBatch of nb instances
+// of one grid size and model; grid.z selects the instance.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "chns_kernels.hpp"
+
+namespace chns {
+namespace passes {
+
+template <int L, int PP = 8> constexpr int ceil_div(int a) { return (a + ColCfg<L, PP>::C - 1) / ColCfg<L, PP>::C; }
+inline int round16(int n) { return (n + 15) / 16 * 16; }
+inline int groupsC(int model) { return model == MODEL_TERNARY ? 5 : (model == MODEL_BINARY ? 4 : 2); }
+
+template <int L, class A> int colA(const A& a, int Jt, int nph, int nb, hipStream_t s) {
+  k_colA<L, A><<<dim3(ceil_div<L, 16>(Jt), nph, nb), ColCfg<L, 16>::NT, 0, s>>>(a);
+  return (int)hipGetLastError();
+}
+template <int L, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
+  const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
+  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else k_rowB<L, MODEL_BINARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  return (int)hipGetLastError();
+}
+template <int L, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
+  const dim3 g(round16(ceil_div<L>(Jt) * groupsC(model)), 1, nb);
+  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else k_colC<L, MODEL_NS, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  return (int)hipGetLastError();
+}
+template <int L, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
+  const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
+  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else k_rowD<L, MODEL_NS, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  return (int)hipGetLastError();
+}
+template <int L, int ST, class A> int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
+  const dim3 g(round16(ceil_div<L>((1 << L) / 2 + 1) * nf), 1, nb);
+  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else k_colE<L, MODEL_NS, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  return (int)hipGetLastError();
+}
+template <int L, class A> int colE(int model, int stage, const A& a, int nf, int nb, hipStream_t s) {
+  return stage == 1 ? colE_st<L, 1>(model, a, nf, nb, s) : colE_st<L, 2>(model, a, nf, nb, s);
+}
+
+}  // namespace passes
+
+#define CHNS_DISPATCH(log2n, CALL)                  \
+  switch (log2n) {                                  \
+    case 4: { constexpr int L = 4; return CALL; }   \
+    case 5: { constexpr int L = 5; return CALL; }   \
+    case 6: { constexpr int L = 6; return CALL; }   \
+    case 7: { constexpr int L = 7; return CALL; }   \
+    case 8: { constexpr int L = 8; return CALL; }   \
+    case 9: { constexpr int L = 9; return CALL; }   \
+    case 10: { constexpr int L = 10; return CALL; } \
+    case 11: { constexpr int L = 11; return CALL; } \
+    default: return (int)hipErrorInvalidValue;      \
+  }
+
+}  // namespace chns
