@@ -292,7 +292,13 @@ def main():
         ctx.step(a.steps)
         tm = ctx.timing_read()
         ctx.timing(False)
+        # host boundary: download and upload of the three fields (PCIe + layout transpose)
+        t2 = time.perf_counter()
         state = ctx.get_state()
+        t3 = time.perf_counter()
+        ctx.set_state(*state)
+        ctx.sync()
+        t4 = time.perf_counter()
     finite = all(bool(np.all(np.isfinite(f))) for f in state)
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     sps = a.steps / (t1 - t0)
@@ -328,6 +334,7 @@ def main():
                                                     f"{FETCH_SCALE:g}/{WRITE_SCALE:g} by a D2D copy"},
         "pass_ms_per_step": {k: tm[k][0] / a.steps for k in tm},
         "kernel_ms_per_step": sum(tm[k][0] for k in tm) / a.steps,
+        "host_boundary_ms": {"get_state": (t3 - t2) * 1e3, "set_state": (t4 - t3) * 1e3},
         "finite": finite,
     }
     if not a.no_cpu_baseline:
