@@ -63,7 +63,7 @@ def measure_traffic(N: int, tag: str, timeout: int = 120):
         try:
             cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
                    sys.executable, os.path.abspath(__file__), "--steps", "3", "--warmup", "1",
-                   "--N", str(N), "--no-cpu-baseline", "--no-traffic"]
+                   "--N", str(N), "--no-cpu-baseline", "--no-traffic", "--adjoint-steps", "0"]
             try:
                 subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=timeout)
             except subprocess.TimeoutExpired:
@@ -217,6 +217,48 @@ def ensemble_bench(steps: int, warmup: int):
     }
 
 
+def adjoint_bench(N: int, nsteps: int, ckpt_every: int):
+    """Adjoint/forward time ratio of the discrete adjoint on the bench workload:
+    forward nsteps with a checkpoint every ckpt_every steps, the c1 shell-spectrum
+    loss against a target from a perturbed run, and the reverse sweep (segment
+    replay + two vector-Jacobian products per step).  Both legs are timed with
+    the state resident in HBM and bracketed by device synchronisations."""
+    import numpy as np
+
+    import chns_amd as C
+    from chns_amd import ic
+
+    kw = workload(N)
+    p = C.params("ternary", **kw)
+    c1, c2 = ic.lens(N, N, kw["lx"], kw["ly"], kw["epsilon"])
+    with C.Context(p, device=0) as ctx:
+        # target: the spectrum of a run from a slightly shifted initial field
+        ctx.set_real_state(None, np.roll(c1, 3, axis=1), np.roll(c2, 3, axis=1))
+        ctx.step(nsteps)
+        target = ctx.spectrum(truncate_state=False)["sph1"]
+        ctx.set_real_state(None, c1, c2)
+        ctx.adj_forward(min(nsteps, 2 * ckpt_every), ckpt_every)   # warm-up (allocations, code objects)
+        ctx.adj_seed_spectrum(target)
+        ctx.adj_backward()
+        ctx.set_real_state(None, c1, c2)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.adj_forward(nsteps, ckpt_every)
+        ctx.sync()
+        t1 = time.perf_counter()
+        loss, _ = ctx.adj_seed_spectrum(target)
+        t2 = time.perf_counter()
+        gw, g1, g2, gs = ctx.adj_backward()
+        t3 = time.perf_counter()
+        ctx.adj_release()
+    fwd, bwd = t1 - t0, t3 - t2
+    return {"workload": f"ternary lens {N}^2, {nsteps} steps, checkpoint every {ckpt_every}, "
+                        "loss = sum_k (S11(k) - S_target(k))^2",
+            "forward_s": fwd, "backward_s": bwd, "adjoint_fwd_ratio": bwd / fwd, "loss": loss,
+            "grad_sigma": gs.tolist(),
+            "finite": bool(np.all(np.isfinite(g1)) and np.all(np.isfinite(g2)) and np.all(np.isfinite(gs)))}
+
+
 def copy_probe(nbytes: int, reps: int = 4):
     """Device-to-device copies of a known size (hipMemcpy) for calibrating the
     FETCH_SIZE / WRITE_SIZE counters against a plain copy kernel."""
@@ -246,6 +288,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--traffic-pass", default="rowD", help="pass whose HBM traffic is counted")
     ap.add_argument("--ensemble", action="store_true", help="measure the batched-ensemble workload instead")
+    ap.add_argument("--adjoint-steps", type=int, default=2000, help="adjoint leg: forward steps (0: skip)")
+    ap.add_argument("--adjoint-ckpt", type=int, default=100, help="adjoint leg: checkpoint interval")
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe(a.copy_probe)
@@ -337,6 +381,8 @@ def main():
         "host_boundary_ms": {"get_state": (t3 - t2) * 1e3, "set_state": (t4 - t3) * 1e3},
         "finite": finite,
     }
+    if a.adjoint_steps > 0:
+        out["adjoint"] = adjoint_bench(N, a.adjoint_steps, a.adjoint_ckpt)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, kw)
     print(json.dumps(out))

@@ -155,6 +155,41 @@ int chns_ensemble_tune(chns_ensemble *e, int reps, int *splits);
  * returns the number of groups. */
 int chns_ensemble_groups(const chns_ensemble *e, int *log2n, int *model, int *count);
 
+/* ---- discrete adjoint of the ternary step (no reference counterpart: the
+ * reference only integrates forward; this differentiates etd2rk() of
+ * etd2rk.c:14-224 with everything it calls) ------------------------------
+ *
+ * Usage: set the initial state, chns_adj_forward (the state ends at the final
+ * time), seed dL/d(final state) with chns_adj_seed_spectrum or chns_adj_seed,
+ * then chns_adj_backward.  Gradients are with respect to the real fields that
+ * chns_set_real_state turns into the initial state, and to sigma_12/13/23.
+ * Spectral adjoints follow one convention: Xbar is the unnormalised r2c of
+ * the real-space gradient dL/dx of x = ifft(X) (fft.c:16-31 scaling), so
+ * dL = sum_x xbar dx.  TERNARY only; storage lives on the device until
+ * chns_adj_release or chns_destroy. */
+
+/* Advance nsteps ETD2RK steps from the current state like chns_step, storing
+ * the state every ckpt_every steps on the device (checkpoints for the reverse
+ * sweep, which replays one segment of ckpt_every steps at a time). */
+int chns_adj_forward(chns_ctx *ctx, int nsteps, int ckpt_every);
+/* Loss = sum_s (sph1(s) - target(s))^2 over the nshell shells of the phase-1
+ * spectrum of spectrum.c:83-103 at the current (final) state, and its seed.
+ * loss and sph1 (nshell entries) may be NULL.  The state is not truncated. */
+int chns_adj_seed_spectrum(chns_ctx *ctx, const double *target, double *loss, double *sph1);
+/* Seed with caller-given spectral adjoints of the final state (IDXC layout,
+ * NULL = zero); they are truncated to the kept modes. */
+int chns_adj_seed(chns_ctx *ctx, const double *wbar, const double *c1bar, const double *c2bar);
+/* Reverse sweep.  g_wz, g_phi1, g_phi2: packed real [Nx][Ny] gradients with
+ * respect to the initial real fields; g_sigma[3]: dL/d(sigma_12, sigma_13,
+ * sigma_23).  Any output may be NULL.  The context state is unchanged. */
+int chns_adj_backward(chns_ctx *ctx, double *g_wz, double *g_phi1, double *g_phi2, double *g_sigma);
+/* After chns_adj_backward: spectral adjoints of the initial state (IDXC) and
+ * the 15 coefficient partial sums behind g_sigma (fac, G1..3, 1/g1..3, mc1,
+ * mc2, A11, A12, A21, A22, lambda/g1, lambda/g2).  Any output may be NULL. */
+int chns_adj_get(chns_ctx *ctx, double *wbar, double *c1bar, double *c2bar, double *partials);
+/* Free the adjoint's device storage (checkpoints, trajectory, work arrays). */
+int chns_adj_release(chns_ctx *ctx);
+
 /* ---- host-only helpers (no device needed) ---------------------------- */
 
 /* ETD2RK coefficients per mode, IDXC order [Nx][Ny2] each, in the exact

@@ -44,6 +44,8 @@ def build(verbose: bool = False) -> str:
         "device": os.path.join(BUILD, "chns_device.o"),
         "batch": os.path.join(BUILD, "chns_batch.o"),
         "ensemble": os.path.join(BUILD, "chns_ensemble.o"),
+        "adjapi": os.path.join(BUILD, "chns_adjoint_api.o"),
+        "adjdev": os.path.join(BUILD, "chns_adjoint.o"),
     }
     steps = [
         ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INC,
@@ -58,11 +60,15 @@ def build(verbose: bool = False) -> str:
          "-c", os.path.join(CSRC, "chns_device.hip"), "-o", objs["device"]],
         [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
          "-c", os.path.join(CSRC, "chns_batch.hip"), "-o", objs["batch"]],
+        [HIPCC, "-O2", "-fPIC", "-std=c++17", "-I" + INC, "-I/opt/rocm/include", "-x", "c++", "-D__HIP_PLATFORM_AMD__",
+         "-c", os.path.join(CSRC, "chns_adjoint.cpp"), "-o", objs["adjapi"]],
+        [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
+         "-c", os.path.join(CSRC, "chns_adjoint.hip"), "-o", objs["adjdev"]],
     ]
-    with ThreadPoolExecutor(6) as ex:
+    with ThreadPoolExecutor(8) as ex:
         list(ex.map(_run, steps))
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["batch"],
-          objs["api"], objs["diag"], objs["ensemble"], objs["tables"]])
+          objs["adjdev"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
     # time-loop driver (plain C on the C ABI), found next to the library via rpath
     _run(["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-I" + INC, os.path.join(HERE, "host", "chns2d.c"),
           "-o", DRIVER, "-L" + os.path.dirname(OUT), "-lchns_amd", "-Wl,-rpath,$ORIGIN", "-lm"])

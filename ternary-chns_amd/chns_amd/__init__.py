@@ -33,8 +33,12 @@ EXPORTS = (
     "chns_free_energy", "chns_kinetic_energy", "chns_perimeter", "chns_spectrum", "chns_real_field",
     "chns_diag_sizes", "chns_ensemble_create", "chns_ensemble_destroy", "chns_ensemble_last_error",
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
-    "chns_ensemble_tune", "chns_ensemble_groups",
+    "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
+    "chns_adj_backward", "chns_adj_get", "chns_adj_release",
 )
+# coefficient partial sums behind the sigma gradient (chns_adj_get)
+ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
+                "A11", "A12", "A21", "A22", "lg1", "lg2")
 PASS_NAMES = ("colA", "rowB", "colC", "rowD", "colE1", "colE2")
 
 
@@ -104,6 +108,12 @@ def load_library(path: str = LIB_PATH):
     lib.chns_spectrum.argtypes = [V, I, D, D, D, D, D, D, D]
     lib.chns_real_field.argtypes = [V, I, I, D]
     lib.chns_diag_sizes.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    lib.chns_adj_forward.argtypes = [V, I, I]
+    lib.chns_adj_seed_spectrum.argtypes = [V, D, D, D]
+    lib.chns_adj_seed.argtypes = [V, D, D, D]
+    lib.chns_adj_backward.argtypes = [V, D, D, D, D]
+    lib.chns_adj_get.argtypes = [V, D, D, D, D]
+    lib.chns_adj_release.argtypes = [V]
     _lib = lib
     return lib
 
@@ -257,6 +267,49 @@ class Context:
         out = np.empty((self.Nx, self.Ny))
         self._check(self.lib.chns_real_field(self.h, int(which), int(bool(truncate)), _dp(out)), "chns_real_field")
         return out
+
+    # -- discrete adjoint (ternary) ---------------------------------------
+    def adj_forward(self, nsteps: int, ckpt_every: int = 100):
+        """Advance nsteps steps storing the state every ckpt_every steps (for the reverse sweep)."""
+        self._check(self.lib.chns_adj_forward(self.h, int(nsteps), int(ckpt_every)), "chns_adj_forward")
+
+    def adj_seed_spectrum(self, target):
+        """Loss sum_s (sph1(s) - target(s))^2 at the current state and its adjoint seed.
+        Returns (loss, sph1)."""
+        ns, nxy = ctypes.c_int(), ctypes.c_int()
+        self.lib.chns_diag_sizes(ctypes.byref(self.p), ctypes.byref(ns), ctypes.byref(nxy))
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        if t.shape != (ns.value,):
+            raise ValueError(f"target needs {ns.value} shells")
+        loss = ctypes.c_double()
+        S = np.zeros(ns.value)
+        self._check(self.lib.chns_adj_seed_spectrum(self.h, _dp(t), ctypes.byref(loss), _dp(S)),
+                    "chns_adj_seed_spectrum")
+        return loss.value, S
+
+    def adj_seed(self, wbar=None, c1bar=None, c2bar=None):
+        """Seed with spectral adjoints of the final state (IDXC complex, None = zero)."""
+        arrs = [None if a is None else self._cplx(a) for a in (wbar, c1bar, c2bar)]
+        self._check(self.lib.chns_adj_seed(self.h, *[_dp(a) for a in arrs]), "chns_adj_seed")
+
+    def adj_backward(self):
+        """Reverse sweep: (g_wz, g_phi1, g_phi2, g_sigma[s12, s13, s23]) with the real-space
+        gradients [Nx][Ny] with respect to the initial real fields."""
+        g = [np.empty((self.Nx, self.Ny)) for _ in range(3)]
+        gs = np.zeros(3)
+        self._check(self.lib.chns_adj_backward(self.h, *[_dp(a) for a in g], _dp(gs)), "chns_adj_backward")
+        return g[0], g[1], g[2], gs
+
+    def adj_get(self):
+        """Spectral adjoints of the initial state (IDXC) and the coefficient partials."""
+        outs = [np.empty((self.Nx, self.Ny2), np.complex128) for _ in range(3)]
+        part = np.zeros(len(ADJ_PARTIALS))
+        self._check(self.lib.chns_adj_get(self.h, *[_dp(o.view(np.float64)) for o in outs], _dp(part)),
+                    "chns_adj_get")
+        return outs, dict(zip(ADJ_PARTIALS, part.tolist()))
+
+    def adj_release(self):
+        self._check(self.lib.chns_adj_release(self.h), "chns_adj_release")
 
     # -- measurement ------------------------------------------------------
     def timing(self, enable: bool = True):

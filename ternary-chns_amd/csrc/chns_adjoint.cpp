@@ -1,0 +1,393 @@
+// C ABI of the discrete adjoint of the ternary ETD2RK step: forward run with
+// state checkpoints, loss seeds, and the reverse sweep with per-segment
+// replay (recompute) of the trajectory.  No reference counterpart: the
+// reference (etd2rk.c, main.c:260-368) only integrates forward.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chns.h"
+#include "chns_adjoint_types.hpp"
+#include "chns_adjoint_launch.hpp"
+#include "chns_ctx.hpp"
+#include "chns_launch.hpp"
+
+struct chns_adj_state {
+  int nsteps = 0, K = 0, nseg = 0;
+  bool forward_done = false, seeded = false, backward_done = false;
+  size_t S = 0;                  // complex entries per spectral field
+  chns::cd* ckpt = nullptr;      // [nseg][3][S]: the state at steps 0, K, 2K, ...
+  chns::cd* traj = nullptr;      // [2K+1][3][S]: u_0..u_K of a segment, then a_0..a_{K-1}
+  int traj_K = 0, ckpt_n = 0;    // allocated capacities
+  chns::cd *ub = nullptr, *ab = nullptr, *gi = nullptr;   // [3][S] each
+  chns::cd* tj = nullptr;        // [6][N*ldJ]: Hxi, Hxix, Het1, Het2, Hmb1, Hmb2
+  chns::cd* tf = nullptr;        // [12][N*ldJ]: O0..O9, Ocb1, Ocb2
+  double* part = nullptr;        // [2][ADJ_NQ][mb]
+  double* acc = nullptr;         // [ADJ_NQ]
+  double* dS = nullptr;          // [nshell]
+  int* nbq = nullptr;            // [ADJ_NQ]
+  int mb = 0, nsh = 0;
+};
+
+namespace {
+
+#define AHIP(ctx, call)                                                  \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) return chns_hip_fail((ctx), e_, #call);       \
+  } while (0)
+#define ALAUNCH(ctx, call)                                               \
+  do {                                                                   \
+    int e_ = (call);                                                     \
+    if (e_ != 0) return chns_hip_fail((ctx), (hipError_t)e_, #call);    \
+  } while (0)
+
+template <class T> int amalloc(chns_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(T));
+  if (e != hipSuccess) return chns_hip_fail(c, e, "hipMalloc(adjoint)");
+  e = hipMemset(q, 0, count * sizeof(T));
+  if (e != hipSuccess) return chns_hip_fail(c, e, "hipMemset(adjoint)");
+  *p = (T*)q;
+  return 0;
+}
+template <class T> void afree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+int nshell_of(int N) { return (int)std::lround(((double)N / std::sqrt(2.0)) + 1.0); }   // main.c:60
+
+chns::cd* field(chns::cd* base, size_t S, int k, int f) { return base + ((size_t)k * 3 + f) * S; }
+
+// Fixed-size work buffers (allocated once per context).
+int ensure_work(chns_ctx* c) {
+  chns_adj_state* a = c->adj;
+  if (a->ub) return 0;
+  const int N = c->d.N, L = c->d.log2n;
+  a->S = (size_t)N * c->d.H;
+  const size_t RJ = (size_t)N * c->ldJ;
+  int rc;
+  if ((rc = amalloc(c, &a->ub, 3 * a->S)) || (rc = amalloc(c, &a->ab, 3 * a->S)) ||
+      (rc = amalloc(c, &a->gi, 3 * a->S)) || (rc = amalloc(c, &a->tj, 6 * RJ)) ||
+      (rc = amalloc(c, &a->tf, 12 * RJ)))
+    return rc;
+  const int nbr = chns::adj_row_blocks(L), nbc = chns::adj_colBC_blocks(L, c->Jt);
+  a->mb = nbr > nbc ? nbr : nbc;
+  a->nsh = nshell_of(N);
+  if ((rc = amalloc(c, &a->part, 2 * (size_t)chns::ADJ_NQ * a->mb)) || (rc = amalloc(c, &a->acc, chns::ADJ_NQ)) ||
+      (rc = amalloc(c, &a->dS, a->nsh)) || (rc = amalloc(c, &a->nbq, chns::ADJ_NQ)))
+    return rc;
+  int nb[chns::ADJ_NQ];
+  for (int q = 0; q < chns::ADJ_NQ; ++q) nb[q] = nbr;
+  nb[chns::AQ_MC1] = nb[chns::AQ_MC2] = nb[chns::AQ_LG1] = nb[chns::AQ_LG2] = nbc;
+  AHIP(c, hipMemcpy(a->nbq, nb, sizeof(nb), hipMemcpyHostToDevice));
+  return 0;
+}
+
+chns::AdjArgs adj_args(chns_ctx* c, const chns::cd* const* x, int slot) {
+  chns_adj_state* a = c->adj;
+  const size_t RJ = (size_t)c->d.N * c->ldJ;
+  chns::AdjArgs r{};
+  chns::AdjBufs& A = r.A;
+  const chns::Bufs& B = c->b;
+  if (x) {
+    A.W = x[0];
+    A.C1 = x[1];
+    A.C2 = x[2];
+  }
+  A.Hc1 = B.Hc1; A.Hc2 = B.Hc2; A.Hu = B.Hu; A.Hv = B.Hv; A.Hwx = B.Hwx; A.Hwy = B.Hwy;
+  A.Hmu1 = B.Hmu1; A.Hmu2 = B.Hmu2; A.Hdx1 = B.Hdx1; A.Hdx2 = B.Hdx2;
+  A.Mh1 = B.Mh1;
+  A.Mh2 = B.Mh2;
+  for (int f = 0; f < 3; ++f) {
+    A.Ub[f] = a->ub + f * a->S;
+    A.Ab[f] = a->ab + f * a->S;
+    A.Gi[f] = a->gi + f * a->S;
+  }
+  A.Hxi = a->tj;
+  A.Hxix = a->tj + RJ;
+  A.Het1 = a->tj + 2 * RJ;
+  A.Het2 = a->tj + 3 * RJ;
+  A.Hmb1 = a->tj + 4 * RJ;
+  A.Hmb2 = a->tj + 5 * RJ;
+  for (int k = 0; k < 10; ++k) A.O[k] = a->tf + k * RJ;
+  A.Ocb1 = a->tf + 10 * RJ;
+  A.Ocb2 = a->tf + 11 * RJ;
+  A.part = a->part + (size_t)slot * chns::ADJ_NQ * a->mb;
+  A.mb = a->mb;
+  A.tw = B.tw;
+  A.tabw1 = B.tabw1;
+  A.tabc1 = B.tabc1;
+  A.tabw2 = B.tabw2;
+  A.tabc2 = B.tabc2;
+  A.amask = B.amask;
+  r.P = c->kp;
+  return r;
+}
+
+chns::Bufs bufs_at(chns_ctx* c, chns::cd* const* in, chns::cd* const* out) {
+  chns::Bufs b = c->b;
+  b.W = in[0];
+  b.C1 = in[1];
+  b.C2 = in[2];
+  b.Wo = out ? out[0] : in[0];
+  b.C1o = out ? out[1] : in[1];
+  b.C2o = out ? out[2] : in[2];
+  return b;
+}
+
+// J_N(x)^T applied to the input left in Gi/Hxi/... by the previous adj_colE,
+// finished by adj_colE<stage>.  Partials go to slot (2 - stage).
+int vjp(chns_ctx* c, chns::cd* const* x, int stage) {
+  const int L = c->d.log2n, M = CHNS_MODEL_TERNARY;
+  hipStream_t s = c->stream;
+  const chns::Bufs b = bufs_at(c, x, nullptr);
+  ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
+  ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+  const chns::cd* xc[3] = {x[0], x[1], x[2]};
+  const chns::AdjArgs A = adj_args(c, xc, 2 - stage);
+  ALAUNCH(c, chns::launch_adj_rowD(L, A, s));
+  ALAUNCH(c, chns::launch_adj_colBC(L, A, s));
+  ALAUNCH(c, chns::launch_adj_rowB(L, A, s));
+  ALAUNCH(c, chns::launch_adj_colE(L, stage, A, s));
+  return 0;
+}
+
+// one forward ETD2RK stage of the replay: state in -> out (pass E out of place)
+int replay_stage(chns_ctx* c, chns::cd* const* in, chns::cd* const* out, int stage, bool need_colA) {
+  const int L = c->d.log2n, M = CHNS_MODEL_TERNARY;
+  hipStream_t s = c->stream;
+  const chns::Bufs b = bufs_at(c, in, out);
+  if (need_colA) ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
+  ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_rowD(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_colE(L, M, stage, b, c->kp, 3, s));
+  return 0;
+}
+
+int check_ctx(chns_ctx* c) {
+  if (!c) return CHNS_E_ARG;
+  if (c->p.model != CHNS_MODEL_TERNARY) return chns_fail(c, CHNS_E_ARG, "the adjoint supports the ternary model");
+  if (c->in_ensemble) return chns_fail(c, CHNS_E_ARG, "the adjoint runs on a stand-alone context");
+  AHIP(c, hipSetDevice(c->dev));
+  if (!c->adj) c->adj = new chns_adj_state();
+  return ensure_work(c);
+}
+
+// dL/dsigma from the coefficient partials (gamma_i, del of main.c:166-170;
+// fac = 2 del/eps, mc_i = 0.75 g_i eps, A_ij and lambda/g_i of the forward step)
+void sigma_gradient(const chns_ctx* c, const double* q, double* out) {
+  const double g1 = c->d.g1, g2 = c->d.g2, g3 = c->d.g3, eps = c->d.eps, lam = c->p.lambda;
+  const double D = g1 * g2 + g1 * g3 + g2 * g3, P3 = g1 * g2 * g3;
+  const double dd1 = 6.0 * g2 * g3 / D - 6.0 * P3 * (g2 + g3) / (D * D);
+  const double dd2 = 6.0 * g1 * g3 / D - 6.0 * P3 * (g1 + g3) / (D * D);
+  const double dd3 = 6.0 * g1 * g2 / D - 6.0 * P3 * (g1 + g2) / (D * D);
+  using namespace chns;
+  const double rf = q[AQ_FAC] * 2.0 / eps;
+  const double dg1 = q[AQ_G1] - q[AQ_G1I] / (g1 * g1) + 0.75 * eps * q[AQ_MC1] -
+                     g3 / (g1 * g1) * (q[AQ_A11] + q[AQ_A21]) - lam / (g1 * g1) * q[AQ_LG1] + rf * dd1;
+  const double dg2 = q[AQ_G2] - q[AQ_G2I] / (g2 * g2) + 0.75 * eps * q[AQ_MC2] -
+                     g3 / (g2 * g2) * (q[AQ_A12] + q[AQ_A22]) - lam / (g2 * g2) * q[AQ_LG2] + rf * dd2;
+  const double dg3 = q[AQ_G3] - q[AQ_G3I] / (g3 * g3) + (q[AQ_A11] + q[AQ_A21]) / g1 +
+                     (q[AQ_A12] + q[AQ_A22]) / g2 + rf * dd3;
+  // gamma_1 = s12 - s23 + s13, gamma_2 = s12 - s13 + s23, gamma_3 = s23 - s12 + s13
+  out[0] = dg1 + dg2 - dg3;    // sigma_12
+  out[1] = dg1 - dg2 + dg3;    // sigma_13
+  out[2] = -dg1 + dg2 + dg3;   // sigma_23
+}
+
+// after a seed in Ub: zero the partial sums and prepare the first product's input
+int after_seed(chns_ctx* c) {
+  chns_adj_state* a = c->adj;
+  AHIP(c, hipMemsetAsync(a->acc, 0, chns::ADJ_NQ * sizeof(double), c->stream));
+  const chns::AdjArgs A = adj_args(c, nullptr, 0);
+  ALAUNCH(c, chns::launch_adj_colE(c->d.log2n, 0, A, c->stream));
+  a->seeded = true;
+  a->backward_done = false;
+  return 0;
+}
+
+}  // namespace
+
+void chns_adj_free(chns_ctx* c) {
+  if (!c || !c->adj) return;
+  chns_adj_state* a = c->adj;
+  (void)hipSetDevice(c->dev);
+  (void)hipStreamSynchronize(c->stream);
+  afree(a->ckpt);
+  afree(a->traj);
+  afree(a->ub);
+  afree(a->ab);
+  afree(a->gi);
+  afree(a->tj);
+  afree(a->tf);
+  afree(a->part);
+  afree(a->acc);
+  afree(a->dS);
+  afree(a->nbq);
+  delete a;
+  c->adj = nullptr;
+}
+
+extern "C" int chns_adj_release(chns_ctx* c) {
+  if (!c) return CHNS_E_ARG;
+  chns_adj_free(c);
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_forward(chns_ctx* c, int nsteps, int ckpt_every) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_state) return chns_fail(c, CHNS_E_STATE, "no state set");
+  if (nsteps < 0 || ckpt_every < 1) return chns_fail(c, CHNS_E_ARG, "nsteps >= 0 and ckpt_every >= 1 required");
+  chns_adj_state* a = c->adj;
+  const int K = nsteps < ckpt_every ? (nsteps > 0 ? nsteps : 1) : ckpt_every;
+  const int nseg = nsteps > 0 ? (nsteps + K - 1) / K : 0;
+  if (a->ckpt_n < nseg) {
+    afree(a->ckpt);
+    a->ckpt_n = 0;
+    if ((rc = amalloc(c, &a->ckpt, (size_t)nseg * 3 * a->S))) return rc;
+    a->ckpt_n = nseg;
+  }
+  if (a->traj_K < K) {
+    afree(a->traj);
+    a->traj_K = 0;
+    if ((rc = amalloc(c, &a->traj, (size_t)(2 * K + 1) * 3 * a->S))) return rc;
+    a->traj_K = K;
+  }
+  a->nsteps = nsteps;
+  a->K = K;
+  a->nseg = nseg;
+  chns::cd* st[3] = {c->b.W, c->b.C1, c->b.C2};
+  for (int sg = 0; sg < nseg; ++sg) {
+    for (int f = 0; f < 3; ++f)
+      AHIP(c, hipMemcpyAsync(field(a->ckpt, a->S, sg, f), st[f], a->S * sizeof(chns::cd), hipMemcpyDeviceToDevice,
+                             c->stream));
+    const int len = (sg + 1) * K <= nsteps ? K : nsteps - sg * K;
+    if ((rc = chns_step(c, len))) return rc;
+  }
+  a->forward_done = true;
+  a->seeded = false;
+  a->backward_done = false;
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_seed_spectrum(chns_ctx* c, const double* target, double* loss, double* sph1) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  chns_adj_state* a = c->adj;
+  if (!a->forward_done) return chns_fail(c, CHNS_E_STATE, "chns_adj_forward first");
+  if (!target) return chns_fail(c, CHNS_E_ARG, "null target spectrum");
+  std::vector<double> S(a->nsh), d(a->nsh);
+  if ((rc = chns_spectrum(c, 0, nullptr, nullptr, S.data(), nullptr, nullptr, nullptr, nullptr))) return rc;
+  double L = 0.0;
+  for (int k = 0; k < a->nsh; ++k) {
+    d[k] = S[k] - target[k];
+    L += d[k] * d[k];
+  }
+  if (loss) *loss = L;
+  if (sph1) std::memcpy(sph1, S.data(), a->nsh * sizeof(double));
+  AHIP(c, hipMemcpyAsync(a->dS, d.data(), a->nsh * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  chns::cd* ubp[3] = {a->ub, a->ub + a->S, a->ub + 2 * a->S};
+  ALAUNCH(c, chns::launch_adj_seed_spectrum(c->b.C1, ubp, a->dS, c->b.amask, c->d.N, c->d.factor_x,
+                                            2.0 * M_PI / c->p.lx, a->nsh, c->stream));
+  if ((rc = after_seed(c))) return rc;
+  AHIP(c, hipStreamSynchronize(c->stream));
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_seed(chns_ctx* c, const double* wbar, const double* c1bar, const double* c2bar) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  chns_adj_state* a = c->adj;
+  if (!a->forward_done) return chns_fail(c, CHNS_E_STATE, "chns_adj_forward first");
+  const int N = c->d.N, H = c->d.H;
+  const double* src[3] = {wbar, c1bar, c2bar};
+  std::vector<double> tmp;
+  for (int f = 0; f < 3; ++f) {
+    chns::cd* dst = a->ub + f * a->S;
+    if (src[f]) {
+      chns_to_cm(src[f], tmp, N, H);
+      AHIP(c, hipMemcpyAsync(dst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
+      AHIP(c, hipStreamSynchronize(c->stream));
+      ALAUNCH(c, chns::launch_truncate(dst, c->b.amask, N, c->stream));   // adjoint states live on the kept modes
+    } else {
+      AHIP(c, hipMemsetAsync(dst, 0, a->S * sizeof(chns::cd), c->stream));
+    }
+  }
+  if ((rc = after_seed(c))) return rc;
+  AHIP(c, hipStreamSynchronize(c->stream));
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_backward(chns_ctx* c, double* g_wz, double* g_phi1, double* g_phi2, double* g_sigma) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  chns_adj_state* a = c->adj;
+  if (!a->seeded) return chns_fail(c, CHNS_E_STATE, "seed the adjoint first");
+  const int K = a->K;
+  hipStream_t s = c->stream;
+  for (int sg = a->nseg - 1; sg >= 0; --sg) {
+    const int len = (sg + 1) * K <= a->nsteps ? K : a->nsteps - sg * K;
+    // replay the segment from its checkpoint, keeping u_k and a_k
+    chns::cd* u[3];
+    chns::cd* av[3];
+    chns::cd* un[3];
+    for (int k = 0; k < len; ++k) {
+      for (int f = 0; f < 3; ++f) {
+        u[f] = k == 0 ? field(a->ckpt, a->S, sg, f) : field(a->traj, a->S, k, f);
+        av[f] = field(a->traj, a->S, K + 1 + k, f);
+        un[f] = field(a->traj, a->S, k + 1, f);
+      }
+      if ((rc = replay_stage(c, u, av, 1, k == 0))) return rc;
+      if (k + 1 < len && (rc = replay_stage(c, av, un, 2, false))) return rc;
+    }
+    // reverse sweep over the segment
+    for (int k = len - 1; k >= 0; --k) {
+      for (int f = 0; f < 3; ++f) {
+        u[f] = k == 0 ? field(a->ckpt, a->S, sg, f) : field(a->traj, a->S, k, f);
+        av[f] = field(a->traj, a->S, K + 1 + k, f);
+      }
+      if ((rc = vjp(c, av, 2)) || (rc = vjp(c, u, 1))) return rc;
+      ALAUNCH(c, chns::launch_adj_accum(a->part, 2, (size_t)chns::ADJ_NQ * a->mb, a->mb, a->nbq, a->acc, s));
+    }
+  }
+  c->hc_valid = false;   // pass A's outputs no longer belong to the current state
+  a->backward_done = true;
+  a->seeded = false;
+  double q[chns::ADJ_NQ];
+  AHIP(c, hipMemcpyAsync(q, a->acc, sizeof(q), hipMemcpyDeviceToHost, s));
+  AHIP(c, hipStreamSynchronize(s));
+  if (g_sigma) sigma_gradient(c, q, g_sigma);
+  double* outs[3] = {g_wz, g_phi1, g_phi2};
+  for (int f = 0; f < 3; ++f)
+    if (outs[f] && (rc = chns_real_of(c, a->ub + f * a->S, outs[f]))) return rc;
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_get(chns_ctx* c, double* wbar, double* c1bar, double* c2bar, double* partials) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  chns_adj_state* a = c->adj;
+  if (!a->backward_done) return chns_fail(c, CHNS_E_STATE, "no completed backward sweep");
+  const int N = c->d.N, H = c->d.H;
+  std::vector<double> tmp((size_t)2 * a->S);
+  double* dst[3] = {wbar, c1bar, c2bar};
+  for (int f = 0; f < 3; ++f) {
+    if (!dst[f]) continue;
+    AHIP(c, hipMemcpyAsync(tmp.data(), a->ub + f * a->S, tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    AHIP(c, hipStreamSynchronize(c->stream));
+    chns_from_cm(tmp, dst[f], N, H);
+  }
+  if (partials) {
+    AHIP(c, hipMemcpyAsync(partials, a->acc, chns::ADJ_NQ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AHIP(c, hipStreamSynchronize(c->stream));
+  }
+  return CHNS_OK;
+}

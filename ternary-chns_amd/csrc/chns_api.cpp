@@ -57,8 +57,10 @@ template <class T> int dalloc(chns_ctx* c, T** p, size_t count) {
   return chns_dalloc_bytes(c, (void**)p, count * sizeof(T));
 }
 
+}  // namespace
+
 // host IDXC [N][H] complex <-> device column-major [H][N]
-void to_cm(const double* src, std::vector<double>& dst, int N, int H) {
+void chns_to_cm(const double* src, std::vector<double>& dst, int N, int H) {
   dst.resize((size_t)2 * N * H);
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < H; ++j) {
@@ -66,13 +68,15 @@ void to_cm(const double* src, std::vector<double>& dst, int N, int H) {
       dst[2 * ((size_t)j * N + i) + 1] = src[2 * ((size_t)i * H + j) + 1];
     }
 }
-void from_cm(const std::vector<double>& src, double* dst, int N, int H) {
+void chns_from_cm(const std::vector<double>& src, double* dst, int N, int H) {
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < H; ++j) {
       dst[2 * ((size_t)i * H + j)] = src[2 * ((size_t)j * N + i)];
       dst[2 * ((size_t)i * H + j) + 1] = src[2 * ((size_t)j * N + i) + 1];
     }
 }
+
+namespace {
 
 void fill_kparams(chns_ctx* c) {
   const chns_params& p = c->p;
@@ -132,6 +136,7 @@ extern "C" const char* chns_last_error(const chns_ctx* ctx) {
 
 extern "C" void chns_destroy(chns_ctx* c) {
   if (!c || c->in_ensemble) return;   // an ensemble's instances go with the ensemble
+  chns_adj_free(c);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* q : c->allocs) (void)hipFree(q);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -187,6 +192,9 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   if ((rc = dalloc(c, &b.W, S)) || (rc = dalloc(c, &b.NW, S))) return bail(rc);
   if (c->nph >= 1 && ((rc = dalloc(c, &b.C1, S)) || (rc = dalloc(c, &b.NC1, S)))) return bail(rc);
   if (c->nph >= 2 && ((rc = dalloc(c, &b.C2, S)) || (rc = dalloc(c, &b.NC2, S)))) return bail(rc);
+  b.Wo = b.W;
+  b.C1o = b.C1;
+  b.C2o = b.C2;
   chns::cd** rj[] = {&b.Hc1, &b.Hc2, &b.Hu, &b.Hv, &b.Hwx, &b.Hwy, &b.Hmu1, &b.Hmu2, &b.Hdx1, &b.Hdx2};
   for (auto q : rj)
     if ((rc = dalloc(c, q, RJ))) return bail(rc);
@@ -230,7 +238,7 @@ extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi
   const double* src[3] = {cwz, cphi1, cphi2};
   chns::cd* dst[3] = {c->b.W, c->b.C1, c->b.C2};
   for (int f = 0; f <= c->nph; ++f) {
-    to_cm(src[f], tmp, N, H);
+    chns_to_cm(src[f], tmp, N, H);
     CHNS_HIP(c, hipMemcpyAsync(dst[f], tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
     CHNS_HIP(c, hipStreamSynchronize(c->stream));
   }
@@ -251,7 +259,7 @@ extern "C" int chns_get_state(chns_ctx* c, double* cwz, double* cphi1, double* c
     if (!dsts[f]) continue;
     CHNS_HIP(c, hipMemcpyAsync(tmp.data(), src[f], tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
     CHNS_HIP(c, hipStreamSynchronize(c->stream));
-    from_cm(tmp, dsts[f], N, H);
+    chns_from_cm(tmp, dsts[f], N, H);
   }
   return CHNS_OK;
 }

@@ -194,6 +194,20 @@ extern "C" int chns_real_field(chns_ctx* c, int which, int truncate, double* out
   return CHNS_OK;
 }
 
+int chns_real_of(chns_ctx* c, const chns::cd* S, double* out) {
+  int rc = prepare(c);
+  if (rc) return rc;
+  const int L = c->d.log2n;
+  DLAUNCH(c, chns::launch_col_inv_full(L, chns::COP_ID, false, S, c->dT1, c->b.tw, c->b.amask, c->ldF,
+                                       c->d.factor_x, c->stream));
+  DLAUNCH(c, chns::launch_row_c2r_full(L, c->dT1, nullptr, real_buf(c, 0), nullptr, c->b.tw, c->ldF,
+                                       1.0 / ((double)c->p.Nx * c->p.Ny), c->stream));
+  const size_t nb = (size_t)c->d.N * c->d.N * sizeof(double);
+  DHIP(c, hipMemcpyAsync(out, real_buf(c, 0), nb, hipMemcpyDeviceToHost, c->stream));
+  DHIP(c, hipStreamSynchronize(c->stream));
+  return CHNS_OK;
+}
+
 extern "C" int chns_diag_sizes(const chns_params* p, int* nshell, int* nxy) {
   if (!p || p->Nx <= 0) return CHNS_E_ARG;
   if (nshell) *nshell = nshell_of(p->Nx);

@@ -33,6 +33,7 @@ struct chns_ctx {
   double* dred = nullptr;                    // reduction partials + result
   double* dspec = nullptr;                   // spectrum row partials
   double* dspec_out = nullptr;               // spectrum sums
+  struct chns_adj_state* adj = nullptr;      // discrete-adjoint storage (chns_adjoint.cpp)
 };
 
 int chns_fail(chns_ctx* c, int code, const std::string& msg);
@@ -40,3 +41,11 @@ int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what);
 int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes);
 // chns_create on a caller-owned stream (nullptr: the context creates its own)
 int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t shared);
+// host IDXC [N][H] complex <-> device column-major [H][N] staging
+void chns_to_cm(const double* src, std::vector<double>& dst, int N, int H);
+void chns_from_cm(const std::vector<double>& src, double* dst, int N, int H);
+// packed real [N][N] c2r (scaled 1/(Nx*Ny)) of a column-major spectral field S
+// of this context, copied to host memory out (synchronises)
+int chns_real_of(chns_ctx* c, const chns::cd* S, double* out);
+// frees the adjoint storage (chns_destroy, chns_adj_release)
+void chns_adj_free(chns_ctx* c);

@@ -591,7 +591,8 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
     }
   }
   // ETD2RK update (etd2rk.c:99,113,129-130 stage 1; :174,188,205-206 stage 2)
-  cd* S = g == 0 ? B.W : (g == 1 ? B.C1 : B.C2);
+  const cd* S = g == 0 ? B.W : (g == 1 ? B.C1 : B.C2);
+  cd* So = g == 0 ? B.Wo : (g == 1 ? B.C1o : B.C2o);
   cd* Nsv = g == 0 ? B.NW : (g == 1 ? B.NC1 : B.NC2);
   if (ok) {
 #pragma unroll
@@ -604,14 +605,14 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
         const cd k1 = (g == 0 ? B.tabw1 : B.tabc1)[ti];
         const double e = k1.x, z = k1.y;
         const cd a = cadd(cscale(u0, e), cscale(v[m], z));
-        S[idx] = a;
+        So[idx] = a;
         Nsv[idx] = v[m];
         s[m] = a;
       } else {
         const double z = (g == 0 ? B.tabw2 : B.tabc2)[ti];
         const cd nn = Nsv[idx];
         const cd a = cadd(u0, cscale(csub(v[m], nn), z));
-        S[idx] = a;
+        So[idx] = a;
         s[m] = a;
       }
     }
