@@ -24,7 +24,7 @@ struct chns_adj_state {
   int traj_K = 0, ckpt_n = 0;    // allocated capacities
   chns::cd *ub = nullptr, *ab = nullptr, *gi = nullptr;   // [3][S] each
   chns::cd* tj = nullptr;        // [6][N*ldJ]: Hxi, Hxix, Het1, Het2, Hmb1, Hmb2
-  chns::cd* tf = nullptr;        // [12][N*ldJ]: O0..O9, Ocb1, Ocb2
+  chns::cd* tf = nullptr;        // [16][N*ldJ]: O0..O13, Ocb1, Ocb2
   double* part = nullptr;        // [2][ADJ_NQ][mb]
   double* acc = nullptr;         // [ADJ_NQ]
   double* dS = nullptr;          // [nshell]
@@ -73,7 +73,7 @@ int ensure_work(chns_ctx* c) {
   int rc;
   if ((rc = amalloc(c, &a->ub, 3 * a->S)) || (rc = amalloc(c, &a->ab, 3 * a->S)) ||
       (rc = amalloc(c, &a->gi, 3 * a->S)) || (rc = amalloc(c, &a->tj, 6 * RJ)) ||
-      (rc = amalloc(c, &a->tf, 12 * RJ)))
+      (rc = amalloc(c, &a->tf, 16 * RJ)))
     return rc;
   const int nbr = chns::adj_row_blocks(L), nbc = chns::adj_colBC_blocks(L, c->Jt);
   a->mb = nbr > nbc ? nbr : nbc;
@@ -114,9 +114,9 @@ chns::AdjArgs adj_args(chns_ctx* c, const chns::cd* const* x, int slot) {
   A.Het2 = a->tj + 3 * RJ;
   A.Hmb1 = a->tj + 4 * RJ;
   A.Hmb2 = a->tj + 5 * RJ;
-  for (int k = 0; k < 10; ++k) A.O[k] = a->tf + k * RJ;
-  A.Ocb1 = a->tf + 10 * RJ;
-  A.Ocb2 = a->tf + 11 * RJ;
+  for (int k = 0; k < 14; ++k) A.O[k] = a->tf + k * RJ;
+  A.Ocb1 = a->tf + 14 * RJ;
+  A.Ocb2 = a->tf + 15 * RJ;
   A.part = a->part + (size_t)slot * chns::ADJ_NQ * a->mb;
   A.mb = a->mb;
   A.tw = B.tw;

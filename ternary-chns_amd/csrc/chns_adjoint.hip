@@ -12,7 +12,10 @@ using passes::ceil_div;
 using passes::round16;
 
 template <int L> int adj_rowD(const AdjArgs& a, hipStream_t s) {
-  k_adj_rowD<L><<<(1 << L) / RowCfg<L>::R, RowCfg<L>::NT, 0, s>>>(a);
+  k_adj_rowD1<L><<<(1 << L) / RowCfg<L>::R, RowCfg<L>::NT, 0, s>>>(a);
+  int e = (int)hipGetLastError();
+  if (e) return e;
+  k_adj_rowD2<L><<<(1 << L) / RowCfg<L>::R, RowCfg<L>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
 template <int L> int adj_colBC(const AdjArgs& a, hipStream_t s) {

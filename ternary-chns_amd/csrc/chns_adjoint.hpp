@@ -14,10 +14,12 @@
 // One vector-Jacobian product J_N(x)^T g at a linearisation point x reuses
 // the forward passes A, B, C at x (chns_kernels.hpp) for the real-space
 // fields c_i, grad c_i, u, v, grad w, mu_i, then:
-//   adj_rowD : row c2r of xi = c2r(g_w), d_y xi, d_x xi, eta_i = c2r(g_i) and
-//              of the forward fields; pointwise transposes of u.grad w,
-//              u.grad c_i and F = A1 grad c1 + A2 grad c2; row r2c of ubar,
-//              vbar, grad-w bars, grad-c bars, mubar (outputs O0..O9)
+//   adj_rowD1: row c2r of xi = c2r(g_w), eta_i = c2r(g_i) and of u, v, grad w,
+//              grad c_i; transposes of u.grad w and u.grad c_i; row r2c of
+//              ubar, vbar, the grad-w bars and the advective grad-c bars
+//   adj_rowD2: row c2r of d_x xi, d_y xi, grad c_i, mu_i; transpose of the
+//              surface-tension force F = A1 grad c1 + A2 grad c2; row r2c of
+//              its grad-c bars and of mubar_i
 //   adj_colBC: column FFT of r2c(mubar_i), Mbar_i = T(.) - (lambda/g_i) k^2 g_i,
 //              column iFFT -> Hmb_i
 //   adj_rowB : row c2r of c_i and muFbar_i, transpose of the chemical potential
@@ -53,51 +55,90 @@ __device__ __forceinline__ void block_sum(const double* s, double* red, double* 
   if (tid < NQ) dst[tid * stride] = red[tid * NT];
 }
 
-// ============================================================ adj_rowD
-// grid: N / R (one or more rows per workgroup)
+// Row-pass input loader shared by the adjoint row kernels: at one row per
+// workgroup (N = 2048) each field pair is staged in LDS by the DMA row stager
+// while the previous pair is transformed; otherwise pairs load directly.
+// Pair k: X_k, Y_k (Y may be null or equal to X), YIK_k: Y multiplied by i ky.
 template <int LOG2N>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
+struct AdjRowLoader {
+  static constexpr bool STG = RowCfg<LOG2N>::R == 1;
+  RowStager<LOG2N> st;
+  size_t ro;
+  int Jt;
+  double f;
+  // first pair: stage and assemble it, then start staging the second
+  template <bool YIK, bool HASY, bool YIK2, bool HASY2>
+  __device__ __forceinline__ void first(const cd* X, const cd* Y, const cd* X2, const cd* Y2, int t, cd* v) const {
+    if constexpr (STG) {
+      const bool same = (X == Y);
+      if (same) st.template issue<false>(X + ro, nullptr);
+      else st.template issue<HASY>(X + ro, HASY ? Y + ro : nullptr);
+      st.wait();
+      if (same) assemble_pair<LOG2N, YIK>(st.stg, st.stg, 1, Jt, f, t, v);
+      else st.template assemble<YIK, HASY>(f, t, v);
+      lds_sync();
+      st.template issue<HASY2>(X2 + ro, HASY2 ? Y2 + ro : nullptr);
+    } else {
+      row_load_pair<LOG2N, YIK>(X + ro, HASY ? Y + ro : nullptr, Jt, f, t, v);
+    }
+  }
+  // pair staged by the previous call (X, Y); start staging (X2, Y2) unless LAST
+  template <bool YIK, bool HASY, bool HASY2, bool LAST>
+  __device__ __forceinline__ void next(const cd* X, const cd* Y, const cd* X2, const cd* Y2, int t, cd* v) const {
+    if constexpr (STG) {
+      st.wait();
+      st.template assemble<YIK, HASY>(f, t, v);
+      if constexpr (!LAST) {
+        lds_sync();
+        st.template issue<HASY2>(X2 + ro, HASY2 ? Y2 + ro : nullptr);
+      }
+    } else {
+      row_load_pair<LOG2N, YIK>(X + ro, HASY ? Y + ro : nullptr, Jt, f, t, v);
+    }
+  }
+};
+
+// ============================================================ adj_rowD1
+// Transposes of the advection terms: N_w = -R(u w_x + v w_y) + ...,
+// N_i = -R(u c_ix + v c_iy) - ...  with nlzbar = -xi, advbar_i = -eta_i
+// (xi = c2r(g_w), eta_i = c2r(g_i)).  Outputs: O0 = -i ky R(ubar), O1 = R(vbar),
+// O2 = R(w_x bar), O3 = -i ky R(w_y bar), and the advective parts of the grad-c
+// bars: O10 = R(c1x bar), O11 = -i ky R(c1y bar), O12, O13 likewise for c2.
+// grid: N / R
+template <int LOG2N>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args) {
   const AdjBufs& A = args.A;
   const KParams& P = args.P;
   using RC = RowCfg<LOG2N>;
-  constexpr int T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  constexpr bool STG = AdjRowLoader<LOG2N>::STG;
+  __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
-  const size_t ro = tix<TJ>(x, 0, P.ldJ);
   const size_t fo = tix<TF>(x, 0, P.ldJ);
   const double sc = P.scale;
   const int Jt = P.Jt;
+  const AdjRowLoader<LOG2N> ld{{stg_all, Jt}, tix<TJ>(x, 0, P.ldJ), Jt, P.f};
   cd v[8];
-  double xi[8], xiy[8], xix[8], e1[8], e2[8];
-  // adjoint inputs: xi, d_y xi (one transform), d_x xi with eta_1, eta_2
-  row_load_pair<LOG2N, true>(A.Hxi + ro, A.Hxi + ro, Jt, P.f, t, v);
+  double xi[8], e1[8], e2[8];
+  ld.template first<false, true, false, false>(A.Hxi, A.Het1, A.Het2, nullptr, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) { xi[m] = v[m].x * sc; xiy[m] = v[m].y * sc; }
-  row_load_pair<LOG2N, false>(A.Hxix + ro, A.Het1 + ro, Jt, P.f, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
-#pragma unroll
-  for (int m = 0; m < 8; ++m) { xix[m] = v[m].x * sc; e1[m] = v[m].y * sc; }
-  row_load_pair<LOG2N, false>(A.Het2 + ro, nullptr, Jt, P.f, t, v);
+  for (int m = 0; m < 8; ++m) { xi[m] = v[m].x * sc; e1[m] = v[m].y * sc; }
+  ld.template next<false, false, true, false>(A.Het2, nullptr, A.Hu, A.Hv, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) e2[m] = v[m].x * sc;
-  // N_w = -R(u w_x + v w_y) + i kx R(Fy) - i ky R(Fx): nlzbar = -xi, Fxbar = d_y xi,
-  // Fybar = -d_x xi;  N_i = -R(u c_ix + v c_iy) - ...: advbar_i = -eta_i
-  double acx1[8], acy1[8], acx2[8], acy2[8];
-  // (u, v): grad-c bars from advection; grad-w bars -> row r2c right away
-  row_load_pair<LOG2N, false>(A.Hu + ro, A.Hv + ro, Jt, P.f, t, v);
+  // (u, v): w bars and the advective grad-c bars, each pair to a row r2c right away
+  double u[8], w[8];
+  ld.template next<false, true, true, false>(A.Hu, A.Hv, A.Hwx, A.Hwy, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const double u = v[m].x * sc, w = v[m].y * sc;
-    acx1[m] = -e1[m] * u;
-    acy1[m] = -e1[m] * w;
-    acx2[m] = -e2[m] * u;
-    acy2[m] = -e2[m] * w;
-    v[m] = mk(-xi[m] * u, -xi[m] * w);   // (w_x bar, w_y bar)
+    u[m] = v[m].x * sc;
+    w[m] = v[m].y * sc;
+    v[m] = mk(-xi[m] * u[m], -xi[m] * w[m]);
   }
   fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
   {
@@ -106,35 +147,55 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
       if (n < Jt) {
         o2[TF * n] = a;
-        o3[TF * n] = ctimes_ik(b, -P.f * (double)n);   // -i ky R(w_y bar)
+        o3[TF * n] = ctimes_ik(b, -P.f * (double)n);
       }
     });
   }
-  double ub[8], vb[8], Ab1[8], Ab2[8];
-  row_load_pair<LOG2N, false>(A.Hwx + ro, A.Hwy + ro, Jt, P.f, t, v);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * u[m], -e2[m] * u[m]);
+  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  {
+    cd* o10 = A.O[10] + fo;
+    cd* o12 = A.O[12] + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+      if (n < Jt) { o10[TF * n] = a; o12[TF * n] = b; }
+    });
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * w[m], -e2[m] * w[m]);
+  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  {
+    cd* o11 = A.O[11] + fo;
+    cd* o13 = A.O[13] + fo;
+    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+      if (n < Jt) {
+        o11[TF * n] = ctimes_ik(a, -P.f * (double)n);
+        o13[TF * n] = ctimes_ik(b, -P.f * (double)n);
+      }
+    });
+  }
+  // ubar = -xi w_x - eta_1 c1x - eta_2 c2x, vbar = -xi w_y - eta_1 c1y - eta_2 c2y
+  double ub[8], vb[8];
+  ld.template next<false, true, true, false>(A.Hwx, A.Hwy, A.Hdx1, A.Hc1, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] = -xi[m] * (v[m].x * sc);
     vb[m] = -xi[m] * (v[m].y * sc);
   }
-  row_load_pair<LOG2N, true>(A.Hdx1 + ro, A.Hc1 + ro, Jt, P.f, t, v);
+  ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const double cx = v[m].x * sc, cy = v[m].y * sc;
-    ub[m] += -e1[m] * cx;
-    vb[m] += -e1[m] * cy;
-    Ab1[m] = xiy[m] * cx - xix[m] * cy;
+    ub[m] += -e1[m] * (v[m].x * sc);
+    vb[m] += -e1[m] * (v[m].y * sc);
   }
-  row_load_pair<LOG2N, true>(A.Hdx2 + ro, A.Hc2 + ro, Jt, P.f, t, v);
+  ld.template next<true, true, false, true>(A.Hdx2, A.Hc2, nullptr, nullptr, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const double cx = v[m].x * sc, cy = v[m].y * sc;
-    ub[m] += -e2[m] * cx;
-    vb[m] += -e2[m] * cy;
-    Ab2[m] = xiy[m] * cx - xix[m] * cy;
+    ub[m] += -e2[m] * (v[m].x * sc);
+    vb[m] += -e2[m] * (v[m].y * sc);
     v[m] = mk(ub[m], vb[m]);
   }
   fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
@@ -148,28 +209,59 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
       }
     });
   }
-  // (mu1, mu2): A_i, the A-coefficient partials, mubar_i, surface-tension grad-c bars
-  double sq[4] = {0.0, 0.0, 0.0, 0.0};
-  double mub1[8], mub2[8];
-  row_load_pair<LOG2N, false>(A.Hmu1 + ro, A.Hmu2 + ro, Jt, P.f, t, v);
+}
+
+// ============================================================ adj_rowD2
+// Transposes of the surface-tension force F = A1 grad c1 + A2 grad c2 with
+// Fxbar = d_y xi, Fybar = -d_x xi: Abar_i = d_y xi c_ix - d_x xi c_iy,
+// mubar (A_i = A_i1 mu_1 + A_i2 mu_2), and the grad-c bars d_y xi A_i, -d_x xi A_i.
+// Outputs: O4 = R(c1x bar), O5 = -i ky R(c1y bar) + mc1 ky^2 R(mu1bar),
+// O6 = R(mu1bar), O7, O8, O9 likewise for c2; partials A11..A22 = sum Abar_i mu_j.
+// grid: N / R
+template <int LOG2N>
+__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args) {
+  const AdjBufs& A = args.A;
+  const KParams& P = args.P;
+  using RC = RowCfg<LOG2N>;
+  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  constexpr bool STG = AdjRowLoader<LOG2N>::STG;
+  __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+  const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
+  const int x = row_of<LOG2N>(r);
+  cd* lds = lds_all + r;
+  const size_t fo = tix<TF>(x, 0, P.ldJ);
+  const double sc = P.scale;
+  const int Jt = P.Jt;
+  const AdjRowLoader<LOG2N> ld{{stg_all, Jt}, tix<TJ>(x, 0, P.ldJ), Jt, P.f};
+  cd v[8];
+  double xix[8], xiy[8], Ab1[8], Ab2[8];
+  ld.template first<true, true, true, true>(A.Hxix, A.Hxi, A.Hdx1, A.Hc1, t, v);
   fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) { xix[m] = v[m].x * sc; xiy[m] = v[m].y * sc; }
+  ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
+  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) Ab1[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
+  ld.template next<true, true, true, false>(A.Hdx2, A.Hc2, A.Hmu1, A.Hmu2, t, v);
+  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+#pragma unroll
+  for (int m = 0; m < 8; ++m) Ab2[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
+  ld.template next<false, true, false, true>(A.Hmu1, A.Hmu2, nullptr, nullptr, t, v);
+  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  double sq[4] = {0.0, 0.0, 0.0, 0.0};
+  double A1[8], A2[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double m1 = v[m].x * sc, m2 = v[m].y * sc;
-    const double A1 = (P.A11 * m1 + P.A12 * m2), A2 = (P.A21 * m1 + P.A22 * m2);
+    A1[m] = (P.A11 * m1 + P.A12 * m2);   // surface_tension.c:144
+    A2[m] = (P.A21 * m1 + P.A22 * m2);   // surface_tension.c:195
     sq[0] += Ab1[m] * m1;
     sq[1] += Ab1[m] * m2;
     sq[2] += Ab2[m] * m1;
     sq[3] += Ab2[m] * m2;
-    mub1[m] = P.A11 * Ab1[m] + P.A21 * Ab2[m];
-    mub2[m] = P.A12 * Ab1[m] + P.A22 * Ab2[m];
-    acx1[m] += xiy[m] * A1;
-    acy1[m] += -xix[m] * A1;
-    acx2[m] += xiy[m] * A2;
-    acy2[m] += -xix[m] * A2;
+    v[m] = mk(xiy[m] * A1[m], xiy[m] * A2[m]);
   }
-#pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = mk(acx1[m], acx2[m]);
   fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o4 = A.O[4] + fo;
@@ -179,7 +271,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
     });
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = mk(acy1[m], mub1[m]);
+  for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A1[m], P.A11 * Ab1[m] + P.A21 * Ab2[m]);
   fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o5 = A.O[5] + fo;
@@ -187,14 +279,13 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
-        const cd d = ctimes_ik(a, -ky);
-        o5[TF * n] = cadd(d, cscale(b, P.mc1 * __dmul_rn(ky, ky)));   // -i ky R(c1y bar) + mc1 ky^2 R(mu1bar)
-        o6[TF * n] = b;                                               // R(mu1bar)
+        o5[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc1 * __dmul_rn(ky, ky)));
+        o6[TF * n] = b;
       }
     });
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = mk(acy2[m], mub2[m]);
+  for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A2[m], P.A12 * Ab1[m] + P.A22 * Ab2[m]);
   fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o8 = A.O[8] + fo;
@@ -202,8 +293,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowD(AdjArgs args) {
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
-        const cd d = ctimes_ik(a, -ky);
-        o8[TF * n] = cadd(d, cscale(b, P.mc2 * __dmul_rn(ky, ky)));
+        o8[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc2 * __dmul_rn(ky, ky)));
         o9[TF * n] = b;
       }
     });
@@ -388,8 +478,14 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], s[m]);
     } else {
       // c_i: T[ R(cbar_i) - i kx R(c_ix bar) - i ky R(c_iy bar) + mc_i k^2 R(mubar_i) ]
-      const int o = g == 1 ? 4 : 7;
+      const int o = g == 1 ? 4 : 7, oa = g == 1 ? 10 : 12;
       col_load_rm<LOG2N, CF::P, TF>(A.O[o], P.ldJ, j, ok, t, s);
+      {
+        cd w[CF::P];
+        col_load_rm<LOG2N, CF::P, TF>(A.O[oa], P.ldJ, j, ok, t, w);
+#pragma unroll
+        for (int m = 0; m < CF::P; ++m) s[m] = cadd(s[m], w[m]);
+      }
       fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
@@ -397,6 +493,9 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
       {
         cd w[CF::P];
         col_load_rm<LOG2N, CF::P, TF>(g == 1 ? A.Ocb1 : A.Ocb2, P.ldJ, j, ok, t, w);
+#pragma unroll
+        for (int m = 0; m < CF::P; ++m) s[m] = cadd(s[m], w[m]);
+        col_load_rm<LOG2N, CF::P, TF>(A.O[oa + 1], P.ldJ, j, ok, t, w);
 #pragma unroll
         for (int m = 0; m < CF::P; ++m) s[m] = cadd(s[m], w[m]);
       }

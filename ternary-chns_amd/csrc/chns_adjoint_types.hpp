@@ -6,7 +6,7 @@
 namespace chns {
 
 // partial sums: fac, G1, G2, G3, g1i, g2i, g3i (adj_rowB), mc1, mc2 (adj_colBC),
-// A11, A12, A21, A22 (adj_rowD), lg1, lg2 (adj_colBC)
+// A11, A12, A21, A22 (adj_rowD2), lg1, lg2 (adj_colBC)
 constexpr int ADJ_NQ = 15;
 enum {
   AQ_FAC = 0, AQ_G1 = 1, AQ_G2 = 2, AQ_G3 = 3, AQ_G1I = 4, AQ_G2I = 5, AQ_G3I = 6, AQ_MC1 = 7, AQ_MC2 = 8,
@@ -19,7 +19,7 @@ struct AdjBufs {
   const cd *Mh1, *Mh2;                             // pass B at it (TF, ldF)
   cd *Ub[3], *Ab[3], *Gi[3];                       // ubar, abar, product input (column-major)
   cd *Hxi, *Hxix, *Het1, *Het2;                    // column iFFTs of the input (TJ, ldJ)
-  cd* O[10];                                       // adj_rowD outputs (TF, ldJ, j < Jt)
+  cd* O[14];                                       // adjoint row-pass outputs (TF, ldJ, j < Jt)
   cd *Hmb1, *Hmb2;                                 // adj_colBC outputs (TJ, ldJ)
   cd *Ocb1, *Ocb2;                                 // adj_rowB outputs (TF, ldJ)
   double* part;                                    // [ADJ_NQ][mb] workgroup partials of this product

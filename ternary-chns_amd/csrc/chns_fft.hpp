@@ -99,6 +99,7 @@ template <int R, bool INV> __device__ __forceinline__ void dftR(cd* a) {
 }
 
 template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw, int idx) {
+  asm volatile("" : "+v"(idx));   // twiddle loads are not shared across transforms (frees their registers)
   cd w = tw[idx];
   return INV ? mk(w.x, -w.y) : w;
 }
@@ -207,6 +208,10 @@ template <int LOG2N, int P, bool INV, int S, bool PRUNED = false>
 __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   static_assert(P == 8 || P == 16, "8 or 16 points per thread");
   static_assert((1 << LOG2N) >= P * 2 && LOG2N <= 12, "supported lengths");
+  // t is made opaque per transform: otherwise the compiler hoists every stage's
+  // LDS and twiddle index arithmetic (a function of t only) out of all the
+  // transforms of a kernel and keeps it live throughout, costing ~100 VGPRs
+  asm volatile("" : "+v"(t));
   lds_sync();
   fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t);
 }
