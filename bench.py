@@ -104,7 +104,9 @@ def pass_bytes(N, Jt, K):
         "colC": 10 * Ft + 3 * Fk,
         "rowD": 10 * Ft + 4 * F,
         "colE1": 15 * F + 2 * Ft,
-        "colE2": 15 * F + 2 * Ft,
+        # stage 2 reads the saved a - z2 N(u) and not the state: 3F less than the
+        # five-pass budget B_alg = 42F + 48Ft + 6Fk, which stays the reported figure
+        "colE2": 12 * F + 2 * Ft,
     }, 42 * F + 48 * Ft + 6 * Fk
 
 

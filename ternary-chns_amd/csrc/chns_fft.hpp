@@ -109,15 +109,39 @@ template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw
 __device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
 template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + (1 << LOG2N) / 8; };
 
+// Table twiddles of one stage for thread t: w, w^2, w^4 (, w^8) of each of
+// its Q = P/R butterflies, w = exp(-+2 pi i k TWS/N), k = b mod NS.  They are
+// loaded one stage ahead, before the LDS exchange that precedes their use, so
+// the load latency hides behind the exchange.
+template <int LOG2N, int P, int NS>
+struct StageTw {
+  static constexpr int N = 1 << LOG2N;
+  static constexpr int R = (N / NS) >= P ? P : (N / NS);
+  static constexpr int Q = P / R;
+  static constexpr int NL = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : 1));
+  cd w[Q][NL];
+  template <bool INV>
+  __device__ __forceinline__ void load(const cd* __restrict__ tw, int t) {
+    if constexpr (NS > 1) {
+      constexpr int T = N / P, TWS = N / (NS * R);
+#pragma unroll
+      for (int u = 0; u < Q; ++u) {
+        const int k = (t + u * T) % NS;
+#pragma unroll
+        for (int l = 0; l < NL; ++l) w[u][l] = twid<INV>(tw, (k << l) * TWS);
+      }
+    }
+  }
+};
+
 // One Stockham stage of radix R on the cyclic registers v[P] of thread t.
 // NS = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
 template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S, bool PRUNED>
-__device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+__device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS>& tws, int t) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / P;
   constexpr int Q = P / R;            // butterflies per thread
-  constexpr int TWS = N / (NS * R);   // twiddle index stride
 #pragma unroll
   for (int u = 0; u < Q; ++u) {
     const int b = t + u * T;
@@ -132,23 +156,23 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const cd* __restrict__ tw,
       for (int m = P / 4 + 1; m < 3 * P / 4; ++m) a[m] = mk(0.0, 0.0);
     }
     if constexpr (NS > 1) {
-      // w^m for m < R from table loads of w, w^2, w^4, w^8, applied as they
-      // are formed so few powers are live at once
-      const cd w1 = twid<INV>(tw, k * TWS);
+      // w^m for m < R from the loaded w, w^2, w^4, w^8, applied as they are
+      // formed so few powers are live at once
+      const cd w1 = tws.w[u][0];
       a[1] = cmulc(a[1], w1);
       if constexpr (R >= 4) {
-        const cd w2 = twid<INV>(tw, 2 * k * TWS);
+        const cd w2 = tws.w[u][1];
         const cd w3 = cmulc(w1, w2);
         a[2] = cmulc(a[2], w2);
         a[3] = cmulc(a[3], w3);
         if constexpr (R >= 8) {
-          const cd w4 = twid<INV>(tw, 4 * k * TWS);
+          const cd w4 = tws.w[u][2];
           a[4] = cmulc(a[4], w4);
           a[5] = cmulc(a[5], cmulc(w1, w4));
           a[6] = cmulc(a[6], cmulc(w2, w4));
           a[7] = cmulc(a[7], cmulc(w3, w4));
           if constexpr (R == 16) {
-            const cd w8 = twid<INV>(tw, 8 * k * TWS);
+            const cd w8 = tws.w[u][3];
             const cd w12 = cmulc(w4, w8);
             a[8] = cmulc(a[8], w8);
             a[9] = cmulc(a[9], cmulc(w1, w8));
@@ -184,16 +208,19 @@ __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
 }
 
 template <int LOG2N, int P, bool INV, int S, int NS, bool PRUNED>
-__device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+__device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ tw, int t,
+                                         const StageTw<LOG2N, P, NS>& tws) {
   constexpr int N = 1 << LOG2N;
   constexpr int R = (N / NS) >= P ? P : (N / NS);
   constexpr bool LAST = NS * R == N;
-  stage<LOG2N, P, R, NS, INV, LAST, S, PRUNED && NS == 1>(v, lds, tw, t);
+  stage<LOG2N, P, R, NS, INV, LAST, S, PRUNED && NS == 1>(v, lds, tws, t);
   if constexpr (!LAST) {
+    StageTw<LOG2N, P, NS * R> nxt;
+    nxt.template load<INV>(tw, t);   // in flight during the exchange
     lds_sync();
     lds_read_cyclic<LOG2N, P, S>(v, lds, t);
     lds_sync();
-    fft_from<LOG2N, P, INV, S, NS * R, false>(v, lds, tw, t);
+    fft_from<LOG2N, P, INV, S, NS * R, false>(v, lds, tw, t, nxt);
   }
 }
 
@@ -213,7 +240,8 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
   // transforms of a kernel and keeps it live throughout, costing ~100 VGPRs
   asm volatile("" : "+v"(t));
   lds_sync();
-  fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t);
+  const StageTw<LOG2N, P, 1> none{};
+  fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t, none);
 }
 
 }  // namespace chns

@@ -594,24 +594,25 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
   const cd* S = g == 0 ? B.W : (g == 1 ? B.C1 : B.C2);
   cd* So = g == 0 ? B.Wo : (g == 1 ? B.C1o : B.C2o);
   cd* Nsv = g == 0 ? B.NW : (g == 1 ? B.NC1 : B.NC2);
+  // Stage 1 keeps b = a - z2 N(u) instead of N(u), so stage 2 forms
+  // u' = a + z2 (N(a) - N(u)) as b + z2 N(a) without reading a back.
   if (ok) {
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
       const size_t idx = (size_t)j * N + i;
       const int ti = j * H + absidx<LOG2N>(i);
-      const cd u0 = S[idx];
+      const double z2 = (g == 0 ? B.tabw2 : B.tabc2)[ti];
       if constexpr (STAGE == 1) {
+        const cd u0 = S[idx];
         const cd k1 = (g == 0 ? B.tabw1 : B.tabc1)[ti];
         const double e = k1.x, z = k1.y;
         const cd a = cadd(cscale(u0, e), cscale(v[m], z));
         So[idx] = a;
-        Nsv[idx] = v[m];
+        Nsv[idx] = csub(a, cscale(v[m], z2));
         s[m] = a;
       } else {
-        const double z = (g == 0 ? B.tabw2 : B.tabc2)[ti];
-        const cd nn = Nsv[idx];
-        const cd a = cadd(u0, cscale(csub(v[m], nn), z));
+        const cd a = cadd(Nsv[idx], cscale(v[m], z2));
         So[idx] = a;
         s[m] = a;
       }

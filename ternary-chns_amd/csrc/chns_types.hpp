@@ -30,7 +30,7 @@ struct Bufs {
   cd *W, *C1, *C2;             // state (column-major [Ny2][N])
   cd *Wo, *C1o, *C2o;          // where pass E writes the updated state: W, C1, C2 themselves
                                // except in the adjoint's trajectory replay
-  cd *NW, *NC1, *NC2;          // N_n saved by stage 1 (etd2rk.c:62-65)
+  cd *NW, *NC1, *NC2;          // stage 1's a - z2 N_n (N_n is saved at etd2rk.c:62-65)
   cd *Hc1, *Hc2;               // row-major [N][ldJ]
   cd *Hu, *Hv, *Hwx, *Hwy, *Hmu1, *Hmu2, *Hdx1, *Hdx2;
   cd *Mh1, *Mh2;               // row-major [N][ldF]
