@@ -123,24 +123,24 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   cd v[8];
   double xi[8], e1[8], e2[8];
   ld.template first<false, true, false, false>(A.Hxi, A.Het1, A.Het2, nullptr, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { xi[m] = v[m].x * sc; e1[m] = v[m].y * sc; }
   ld.template next<false, false, true, false>(A.Het2, nullptr, A.Hu, A.Hv, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) e2[m] = v[m].x * sc;
   // (u, v): w bars and the advective grad-c bars, each pair to a row r2c right away
   double u[8], w[8];
   ld.template next<false, true, true, false>(A.Hu, A.Hv, A.Hwx, A.Hwy, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     u[m] = v[m].x * sc;
     w[m] = v[m].y * sc;
     v[m] = mk(-xi[m] * u[m], -xi[m] * w[m]);
   }
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o2 = A.O[2] + fo;
     cd* o3 = A.O[3] + fo;
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * u[m], -e2[m] * u[m]);
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o10 = A.O[10] + fo;
     cd* o12 = A.O[12] + fo;
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * w[m], -e2[m] * w[m]);
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o11 = A.O[11] + fo;
     cd* o13 = A.O[13] + fo;
@@ -177,28 +177,28 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   // ubar = -xi w_x - eta_1 c1x - eta_2 c2x, vbar = -xi w_y - eta_1 c1y - eta_2 c2y
   double ub[8], vb[8];
   ld.template next<false, true, true, false>(A.Hwx, A.Hwy, A.Hdx1, A.Hc1, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] = -xi[m] * (v[m].x * sc);
     vb[m] = -xi[m] * (v[m].y * sc);
   }
   ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] += -e1[m] * (v[m].x * sc);
     vb[m] += -e1[m] * (v[m].y * sc);
   }
   ld.template next<true, true, false, true>(A.Hdx2, A.Hc2, nullptr, nullptr, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] += -e2[m] * (v[m].x * sc);
     vb[m] += -e2[m] * (v[m].y * sc);
     v[m] = mk(ub[m], vb[m]);
   }
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o0 = A.O[0] + fo;
     cd* o1 = A.O[1] + fo;
@@ -236,19 +236,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   cd v[8];
   double xix[8], xiy[8], Ab1[8], Ab2[8];
   ld.template first<true, true, true, true>(A.Hxix, A.Hxi, A.Hdx1, A.Hc1, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { xix[m] = v[m].x * sc; xiy[m] = v[m].y * sc; }
   ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab1[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<true, true, true, false>(A.Hdx2, A.Hc2, A.Hmu1, A.Hmu2, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab2[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<false, true, false, true>(A.Hmu1, A.Hmu2, nullptr, nullptr, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
   double sq[4] = {0.0, 0.0, 0.0, 0.0};
   double A1[8], A2[8];
 #pragma unroll
@@ -262,7 +262,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
     sq[3] += Ab2[m] * m2;
     v[m] = mk(xiy[m] * A1[m], xiy[m] * A2[m]);
   }
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o4 = A.O[4] + fo;
     cd* o7 = A.O[7] + fo;
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A1[m], P.A11 * Ab1[m] + P.A21 * Ab2[m]);
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o5 = A.O[5] + fo;
     cd* o6 = A.O[6] + fo;
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A2[m], P.A12 * Ab1[m] + P.A22 * Ab2[m]);
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   {
     cd* o8 = A.O[8] + fo;
     cd* o9 = A.O[9] + fo;
@@ -378,11 +378,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
   cd v[8];
   double c1[8], c2[8];
   row_load_pair<LOG2N, false>(A.Hc1 + ro, A.Hc2 + ro, P.Jt, P.f, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { c1[m] = v[m].x * sc; c2[m] = v[m].y * sc; }
   row_load_pair<LOG2N, false>(A.Hmb1 + ro, A.Hmb2 + ro, P.Jt, P.f, t, v);
-  fft<LOG2N, 8, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
   // fac, G1, G2, G3, g1i, g2i, g3i
   double sq[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -422,7 +422,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
                       Gb3 * (P.g3 * fp3 + P.X6 * a1 * a1 * a2 * a2);
     v[m] = mk(d1 - d3, d2 - d3);
   }
-  fft<LOG2N, 8, false, RC::R>(v, lds, A.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
   cd* o1 = A.Ocb1 + tix<TF>(x, 0, P.ldJ);
   cd* o2 = A.Ocb2 + tix<TF>(x, 0, P.ldJ);
   const int Jt = P.Jt;

@@ -66,10 +66,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_c2r_full(const cd* __
   cd v[8];
   const size_t ro = tix<TJ>(x, 0, ldF);
   row_load_pair<LOG2N, false>(X + ro, Y ? Y + ro : nullptr, H, 0.0, t, v);
-  fft<LOG2N, RC::P, true, RC::R>(v, lds, tw, t);
+  rfft<LOG2N, true, RC::R>(v, lds, tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const size_t o = (size_t)x * N + t + m * T;
+    const size_t o = (size_t)x * N + rpos<LOG2N, RC::R>(t, m);
     oa[o] = v[m].x * scale;
     if (ob) ob[o] = v[m].y * scale;
   }

@@ -244,4 +244,175 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
   fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t, none);
 }
 
+
+// ---------------------------------------------------------------------------
+// 2048-point transform on one workgroup of 256 threads (4 waves), 8 points
+// per thread, with a single workgroup barrier between its stages.
+//
+// 2048 = 4 x 512.  Inverse (decimation in frequency): thread t holds
+// X[t + 256 j] (the cyclic spectral layout); with k = m + 512 r, the radix-4
+// DFT over r of X[m + 512 r] for m in {t, t + 256} and the twiddle
+// W^(-m q) are local to the thread; one exchange through LDS gives wave q
+// the sequence z_q[m], and the wave transforms it with a 512-point Stockham
+// FFT whose exchanges stay inside its own quarter of the LDS array (a wave's
+// LDS accesses execute in order, so they need no workgroup barrier).  Lane l
+// of wave q ends with x[4 (l + 64 j) + q].  The forward transform is the
+// transpose (decimation in time): it takes x in that layout and returns
+// X[t + 256 j].  So spectral data keeps the cyclic layout of fft() while
+// real-space points sit at w4_pos(t, j) (see below).
+constexpr int W4_Q = 576;   // LDS entries per wave quarter: lp(511) + 1
+
+__device__ __forceinline__ int w4_pos(int t, int j) { return 4 * (t & 63) + (t >> 6) + 256 * j; }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// 512-point Stockham transform by one wave: lane l holds points l + 64 m
+// (cyclic in, cyclic out); lq is the wave's LDS quarter; twiddles are
+// W_512^e = tw[4 e] of the 2048-point table.
+template <bool INV, bool PRUNED_ZERO_UPPER>
+__device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const cd* __restrict__ tw, int l) {
+  // stage NS = 1 (no twiddles)
+  {
+    cd a[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a[m] = v[m];
+    dft8<INV>(a);
+    const int base = l * 8;
+    cd w1 = twid<INV>(tw, 4 * (l % 8) * 8), w2 = twid<INV>(tw, 8 * (l % 8) * 8), w4 = twid<INV>(tw, 16 * (l % 8) * 8);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lq[lp(base + q)] = a[q];
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = lq[lp(l + 64 * m)];
+    wave_sync();
+    // stage NS = 8: k = l % 8, twiddle exponent k * 8 (W_512), table index 4 k 8
+    a[0] = v[0];
+    a[1] = cmulc(v[1], w1);
+    const cd w3 = cmulc(w1, w2);
+    a[2] = cmulc(v[2], w2);
+    a[3] = cmulc(v[3], w3);
+    a[4] = cmulc(v[4], w4);
+    a[5] = cmulc(v[5], cmulc(w1, w4));
+    a[6] = cmulc(v[6], cmulc(w2, w4));
+    a[7] = cmulc(v[7], cmulc(w3, w4));
+    dft8<INV>(a);
+    const int k = l % 8;
+    const int b2 = (l / 8) * 64 + k;
+    w1 = twid<INV>(tw, 4 * l);
+    w2 = twid<INV>(tw, 8 * l);
+    w4 = twid<INV>(tw, 16 * l);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lq[lp(b2 + q * 8)] = a[q];
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = lq[lp(l + 64 * m)];
+    wave_sync();
+    // stage NS = 64: k = l, twiddle exponent l (W_512), table index 4 l; last stage
+    v[1] = cmulc(v[1], w1);
+    const cd w3b = cmulc(w1, w2);
+    v[2] = cmulc(v[2], w2);
+    v[3] = cmulc(v[3], w3b);
+    v[4] = cmulc(v[4], w4);
+    v[5] = cmulc(v[5], cmulc(w1, w4));
+    v[6] = cmulc(v[6], cmulc(w2, w4));
+    v[7] = cmulc(v[7], cmulc(w3b, w4));
+    dft8<INV>(v);
+  }
+}
+
+// Inverse 2048-point transform (see above).  PRUNED: X[k] = 0 for 512 < k < 1536.
+template <bool PRUNED>
+__device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+  asm volatile("" : "+v"(t));
+  lds_sync();
+  const cd c1 = mk(kR, kR);   // W_2048^(-256) = exp(+i pi/4)
+  const cd w1t = twid<true>(tw, t), w2t = twid<true>(tw, 2 * t);
+  if constexpr (PRUNED) {
+    v[3] = mk(0.0, 0.0);
+    v[4] = mk(0.0, 0.0);
+    v[5] = mk(0.0, 0.0);
+  }
+#pragma unroll
+  for (int jp = 0; jp < 2; ++jp) {
+    const int m = t + 256 * jp;
+    cd a0 = v[jp], a1 = v[jp + 2], a2 = v[jp + 4], a3 = v[jp + 6];
+    dft4<true>(a0, a1, a2, a3);
+    // twiddles W^(-m q): w(m) = w(t) c1^jp, w(2m) = w(2t) (+i)^jp
+    cd wm1 = w1t, wm2 = w2t;
+    if (jp == 1) {
+      wm1 = cmulc(w1t, c1);
+      wm2 = rot90<true>(w2t);
+    }
+    const cd wm3 = cmulc(wm1, wm2);
+    lds[lp(m)] = a0;
+    lds[W4_Q + lp(m)] = cmulc(a1, wm1);
+    lds[2 * W4_Q + lp(m)] = cmulc(a2, wm2);
+    lds[3 * W4_Q + lp(m)] = cmulc(a3, wm3);
+  }
+  lds_sync();
+  const int q = t >> 6, l = t & 63;
+  cd* lq = lds + q * W4_Q;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = lq[lp(l + 64 * j)];
+  wave_sync();
+  fft512_wave<true, false>(v, lq, tw, l);
+}
+
+// Forward 2048-point transform: x in the w4_pos layout -> X[t + 256 j].
+__device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+  asm volatile("" : "+v"(t));
+  lds_sync();
+  const int q = t >> 6, l = t & 63;
+  cd* lq = lds + q * W4_Q;
+  fft512_wave<false, false>(v, lq, tw, l);
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) lq[lp(l + 64 * j)] = v[j];
+  const cd w1t = twid<false>(tw, t), w2t = twid<false>(tw, 2 * t);
+  const cd c1 = mk(kR, -kR);   // W_2048^256 = exp(-i pi/4)
+  lds_sync();
+#pragma unroll
+  for (int jp = 0; jp < 2; ++jp) {
+    const int k = t + 256 * jp;
+    cd wk1 = w1t, wk2 = w2t;
+    if (jp == 1) {
+      wk1 = cmulc(w1t, c1);
+      wk2 = rot90<false>(w2t);
+    }
+    const cd wk3 = cmulc(wk1, wk2);
+    cd b0 = lds[lp(k)];
+    cd b1 = cmulc(lds[W4_Q + lp(k)], wk1);
+    cd b2 = cmulc(lds[2 * W4_Q + lp(k)], wk2);
+    cd b3 = cmulc(lds[3 * W4_Q + lp(k)], wk3);
+    dft4<false>(b0, b1, b2, b3);
+    v[jp] = b0;
+    v[jp + 2] = b1;
+    v[jp + 4] = b2;
+    v[jp + 6] = b3;
+  }
+  lds_sync();   // callers (row_split) write the LDS array next
+}
+
+// Row transform of a kernel with R rows per workgroup (rows interleaved
+// across lanes, 8 points per thread): the 4-wave transform at N = 2048 with
+// one row per workgroup, fft() otherwise.  rpos gives the real-space index of
+// register j of thread t.
+template <int LOG2N, bool INV, int R, bool PRUNED = false>
+__device__ __forceinline__ void rfft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+  if constexpr (LOG2N == 11 && R == 1) {
+    if constexpr (INV) fft_w4_inv<PRUNED>(v, lds, tw, t);
+    else fft_w4_fwd(v, lds, tw, t);
+  } else {
+    fft<LOG2N, 8, INV, R, PRUNED>(v, lds, tw, t);
+  }
+}
+template <int LOG2N, int R>
+__device__ __forceinline__ int rpos(int t, int j) {
+  if constexpr (LOG2N == 11 && R == 1) return w4_pos(t, j);
+  else return t + j * ((1 << LOG2N) / 8);
+}
+
 }  // namespace chns

@@ -278,7 +278,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
   const cd* h1 = B.Hc1 + tix<TJ>(x, 0, P.ldJ);
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix<TJ>(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-  fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
   cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
   cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
@@ -415,14 +415,14 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
     cd* o = B.G1 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
   } else {
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
-      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
-      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         u[m] = v[m].x * sc;
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
-      fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
-        fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+        rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
-    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
     cd* o2 = B.G2 + fo;
     cd* o3 = B.G3 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
@@ -526,10 +526,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     } else {
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     }
-    fft<LOG2N, RC::P, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
-    fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
     cd* o1 = B.G1 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
       const cd iky = ctimes_ik(Bv, P.f * (double)n);
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
-      fft<LOG2N, RC::P, false, RC::R>(v, lds, B.tw, t);
+      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
       cd* o4 = B.G4 + fo;
       row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[TF * n] = A; });
     }
@@ -645,10 +645,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
   cd v[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
-    const size_t o = (size_t)x * N + t + m * T;
+    const size_t o = (size_t)x * N + rpos<LOG2N, RC::R>(t, m);
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
-  fft<LOG2N, RC::P, false, RC::R>(v, lds, tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, tw, t);
   cd* pa = oa + tix<TF>(x, 0, ldF);
   cd* pb = ob ? ob + tix<TF>(x, 0, ldF) : nullptr;
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {

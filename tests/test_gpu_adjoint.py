@@ -199,3 +199,25 @@ def test_torch_autograd_function():
     assert rel(c1.grad.numpy(), 3.0 * grads[1]) < TOL
     assert rel(c2.grad.numpy(), 3.0 * grads[2]) < TOL
     assert np.all(np.abs(sig.grad.numpy() - 3.0 * gsig) <= 1e-9 * np.abs(3.0 * gsig))
+
+
+def test_reverse_step_2048():
+    """One reverse step at 2048^2 (LDS-staged adjoint row passes) against the oracle."""
+    N = 2048
+    kw = dict(Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI, nu=2e-3, dt=1e-6, lam=5e-4, epsilon=1.5 * TWO_PI / N,
+              sigma_12=0.1, sigma_13=0.08, sigma_23=0.06, Xlambda=2.0)
+    op = O.Params(model="ternary", **kw)
+    s0 = O.state_from_real(op, O.random_vorticity(op, 2), *O.init_lens_real(op))
+    rng = np.random.default_rng(5)
+    seeds = [O.truncation(op, O.dfft(rng.standard_normal((N, N)))) for _ in range(3)]
+    ref, part = OA.step_vjp(op, s0, seeds)
+    with C.Context(C.params("ternary", **kw)) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.adj_forward(1, 1)
+        ctx.adj_seed(*seeds)
+        ctx.adj_backward()
+        got, gpart = ctx.adj_get()
+    for a, b in zip(got, ref):
+        assert rel(a, b) < TOL
+    for k in OA.SIGMA_PARTIALS:
+        assert abs(gpart[k] - part[k]) <= 1e-9 * abs(part[k]) + 1e-13 * max(map(abs, part.values())), k

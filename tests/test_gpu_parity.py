@@ -183,3 +183,25 @@ def test_full_size_properties(N):
             assert abs(f[0, 0] - m) <= TOL_MASS * abs(m)
     for a, b in zip(*outs):
         assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_full_size_real_io():
+    """2048^2 real-space boundary: dfft + truncation of real fields and the real dump (the
+    row transforms with one row per workgroup order real-space points differently)."""
+    N = 2048
+    op, cp, _ = case("ternary", N, nu=2e-3, lam=5e-4, dt=1e-6)
+    wz = O.random_vorticity(op, 3)
+    c1, c2 = O.init_lens_real(op)
+    s0 = O.state_from_real(op, wz, c1, c2)
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(wz, c1, c2)
+        g = ctx.get_state()
+        r1 = ctx.real_field(1, truncate=False)
+        r0 = ctx.real_field(0, truncate=True)
+        fe = ctx.free_energy()
+    for a, b in zip(g, s0.fields()):
+        assert rel(a, b) < 1e-14
+    assert rel(r1, O.ifft(op, s0.cphi1)) < 1e-13
+    assert rel(r0, O.ifft(op, O.truncation(op, s0.cwz))) < 1e-13
+    ref = O.free_energy(op, s0)
+    assert all(abs(x - y) <= 1e-12 * abs(y) for x, y in zip(fe, ref))
