@@ -34,7 +34,14 @@ extern "C" {
 /* Runtime form of the reference's compile-time parameters (global.h:20-33,
  * 102-104, 127-134).  `lambda` is the mobility.  epsilon <= 0 selects the
  * reference default 3*dx.  f_amp/f_k: Kolmogorov forcing f_amp*sin(f_k*y) on
- * the vorticity (new: the reference forcing.c does not compile). */
+ * the vorticity (new: the reference forcing.c does not compile).
+ * Optional physics (macro.h switches): gravity - buoyancy torque
+ * i kx F(-g (rho(c) - 1)) of nonlin.c:107-171 with rho1..rho3, g of
+ * global.h:106-110,135-140 (BINARY uses rho1, rho2); vpm - penalised walls
+ * chi(x) of init.c:184-206 with eta = 1.5 dt (global.h:47-48) acting as in
+ * nonlin.c:63-95,209-211,294-296,337-339 and surface_tension.c:83-85,
+ * 151-154,202-205; sponge - the sponge(y)/eta damping of the vorticity
+ * (init.c:207-211, nonlin.c:50-99), which needs vpm. */
 typedef struct {
   int Nx, Ny, model;
   double lx, ly, nu, dt;
@@ -43,6 +50,10 @@ typedef struct {
   int smooth_filter; /* macro.h:15 SMOOTH_FILTER; must be 0 in this build */
   double f_amp;
   int f_k;
+  int gravity;       /* macro.h GRAVITY */
+  double rho1, rho2, rho3, g;
+  int vpm;           /* macro.h VPM */
+  int sponge;        /* macro.h SPONGE (requires vpm) */
 } chns_params;
 
 typedef struct chns_ctx chns_ctx;

@@ -69,6 +69,15 @@ class Params:
     smooth_filter: bool = False   # macro.h:15 SMOOTH_FILTER
     f_amp: float = 0.0            # Kolmogorov forcing amplitude on omega
     f_k: int = 0                  # forcing wavenumber index along y
+    gravity: bool = False         # macro.h GRAVITY (nonlin.c:107-171)
+    rho1: float | None = None     # None -> global.h:108-110 / 137-140 by model
+    rho2: float | None = None
+    rho3: float = 1.0
+    g: float = 3.0
+    vpm: bool = False             # macro.h VPM: penalised walls chi(x) (init.c:184-206)
+    sponge: bool = False          # macro.h SPONGE: sponge(y) on omega (init.c:207-211), needs VPM
+    chi_override: object = None   # test hook: replace chi(x) [Nx] (None: the init.c profile)
+    sponge_override: object = None  # test hook: replace sponge(y) [Ny] (None: the init.c profile)
 
     def __post_init__(self):
         if isinstance(self.model, str):
@@ -77,6 +86,10 @@ class Params:
             self.epsilon = 3.0 * self.dx
         if self.lam is None:
             self.lam = 1e-6 if self.model == MODEL_BINARY else 1e-4
+        if self.rho1 is None:
+            self.rho1 = 1.044 if self.model == MODEL_BINARY else 0.957   # global.h:108,137
+        if self.rho2 is None:
+            self.rho2 = 1.0 if self.model == MODEL_BINARY else 1.044     # global.h:109,138
 
     # derived quantities --------------------------------------------------
     @property
@@ -270,8 +283,9 @@ def surface_tension(p: Params, cphi1, cphi2=None, cmuF1=None, cmuF2=None):
         tux = ifft(p, truncation(p, tcux))                               # :66-71
         tuy = ifft(p, truncation(p, tcuy))
         nphi = ifft(p, truncation(p, ncphi))
-        tux = tux * nphi                                                  # :83-84
-        tuy = tuy * nphi
+        wall = (1.0 - vpm_fields(p)[0])[:, None] if p.vpm else 1.0
+        tux = tux * (nphi * wall)                                         # :83-89
+        tuy = tuy * (nphi * wall)
         return curl(p, dfft(tux), dfft(tuy))                             # :94-97
     g1, g2, g3, _ = p.gammas
     tcphi1 = cmuF1 + _cscale(cphi1, 0.75 * g1 * eps * ks)                # :123
@@ -283,13 +297,14 @@ def surface_tension(p: Params, cphi1, cphi2=None, cmuF1=None, cmuF2=None):
     tphi1 = ifft(p, truncation(p, tcphi1))
     tphi2 = ifft(p, truncation(p, tcphi2))
     A1 = ((1.0 + (g3 / g1)) * tphi1 + (g3 / g2) * tphi2)                 # :144-145
-    out = curl(p, dfft(tux * A1), dfft(tuy * A1))                        # :155-166
+    wall = (1.0 - vpm_fields(p)[0])[:, None] if p.vpm else 1.0
+    out = curl(p, dfft(tux * A1 * wall), dfft(tuy * A1 * wall))          # :151-166
     tcux = _i_times(kx[:, None], cphi2)                                  # :178
     tcuy = _i_times(ky[None, :], cphi2)                                  # :179
     tux = ifft(p, truncation(p, tcux))                                   # :184-187
     tuy = ifft(p, truncation(p, tcuy))
     A2 = ((g3 / g1) * tphi1 + (1.0 + (g3 / g2)) * tphi2)                 # :195-196
-    out = out + curl(p, dfft(tux * A2), dfft(tuy * A2))                  # :205-216
+    out = out + curl(p, dfft(tux * A2 * wall), dfft(tuy * A2 * wall))    # :202-216
     return out
 
 
@@ -308,6 +323,43 @@ def forcing_spectrum(p: Params):
     return F
 
 
+def vpm_fields(p: Params):
+    """chi(x) [Nx] and sponge(y) [Ny] of init.c:184-211 (both x- or y-only), and 1/eta.
+
+    eta = 1.5 dt (global.h:48); eps = 3 dx, walls at xd = 6 dx, xu = lx - xd
+    (likewise in y with dy); sponge includes the factor 1/eta (init.c:210).
+    """
+    etainv = 1.0 / (1.5 * p.dt)
+    eps = 3.0 * p.dx
+    xd, yd = 6.0 * p.dx, 6.0 * p.dy
+    xu, yu = p.lx - xd, p.ly - yd
+    x = np.arange(p.Nx) * p.dx
+    y = np.arange(p.Ny) * p.dy
+    chi = 0.5 * (1.0 - np.tanh(2.0 * (x - xd) / eps)) + 0.5 * (1.0 + np.tanh(2.0 * (x - xu) / eps))
+    sp = (0.5 * (1.0 - np.tanh(2.0 * (y - yd) / eps)) + 0.5 * (1.0 + np.tanh(2.0 * (y - yu) / eps))) * etainv
+    if p.chi_override is not None:
+        chi = np.asarray(p.chi_override, dtype=np.float64)
+    if p.sponge_override is not None:
+        sp = np.asarray(p.sponge_override, dtype=np.float64)
+    return chi, sp, etainv
+
+
+def gravity_term(p: Params, cphi1, cphi2=None):
+    """Buoyancy torque added to cnlz (nonlin.c:107-171): i kx F(-g (rho(c) - 1)) with
+    rho = rho2 + (rho1 - rho2) c (binary) or rho3 + (rho1 - rho3) c1 + (rho2 - rho3) c2
+    (ternary), c from the untruncated state."""
+    kx, _ = kxky(p)
+    if p.model == MODEL_BINARY:
+        t = ifft(p, cphi1)
+        t = (p.rho2 + (p.rho1 - p.rho2) * t) - 1.0
+    else:
+        c1 = ifft(p, cphi1)
+        c2 = ifft(p, cphi2)
+        t = (p.rho3 + (p.rho1 - p.rho3) * c1 + (p.rho2 - p.rho3) * c2) - 1.0
+    t = t * ((-1.0) * p.g)
+    return _i_times(kx[:, None], dfft(t))
+
+
 def nonlin(p: Params, cwz, cphi1=None, cphi2=None, cmuF1=None, cmuF2=None):
     """All nonlinear terms (nonlin.c:13-381).
 
@@ -324,11 +376,20 @@ def nonlin(p: Params, cwz, cphi1=None, cphi2=None, cmuF1=None, cmuF2=None):
     tuy = ifft(p, truncation(p, tcuy))
     nlz = ux * tux + uy * tuy                                            # :62
     cnlz = -dfft(nlz)                                                    # :74,92
+    if p.vpm:
+        chi, sp, etainv = vpm_fields(p)
+        wall = chi[:, None] * etainv
+        cnlz = cnlz - curl(p, dfft(ux * wall), dfft(uy * wall))          # :63-66,76-80,93-95
+        if p.sponge:
+            twz = ifft(p, truncation(p, cwz)) * sp[None, :]              # :50-54,68-70,82-84
+            cnlz = cnlz - dfft(twz)                                      # :97-99
     if p.model == MODEL_NS:
         return cnlz + forcing_spectrum(p)
     if p.model == MODEL_BINARY:
         cphi = cphi1
         cnlz = cnlz + surface_tension(p, cphi)                           # :104
+        if p.gravity:
+            cnlz = cnlz + gravity_term(p, cphi)                          # :108-131
         cnlz = cnlz + forcing_spectrum(p)
         cux, cuy = get_velocity(p, cwz)                                  # :177-179
         tcphi = _i_times(kx[:, None], cphi)                              # :187
@@ -337,7 +398,10 @@ def nonlin(p: Params, cwz, cphi1=None, cphi2=None, cmuF1=None, cmuF2=None):
         nphi = ifft(p, truncation(p, ncphi))
         tux = ifft(p, truncation(p, cux))
         tuy = ifft(p, truncation(p, cuy))
-        cnlphi = dfft(tux * tphi + tuy * nphi)                           # :207,215
+        nlphi = tux * tphi + tuy * nphi                                  # :207
+        if p.vpm:
+            nlphi = nlphi * (1.0 - vpm_fields(p)[0])[:, None]            # :209-211
+        cnlphi = dfft(nlphi)                                             # :215
         tphi = binary_poly(ifft(p, truncation(p, cphi)))                 # :217-228
         tcphi = dfft(tphi)                                               # :230
         tcphi = _cscale(tcphi, 24.0 * (p.sigma / p.epsilon) * ks)       # :240
@@ -345,16 +409,19 @@ def nonlin(p: Params, cwz, cphi1=None, cphi2=None, cmuF1=None, cmuF2=None):
         return cnlz, cnlphi
     g1, g2, g3, _ = p.gammas
     cnlz = cnlz + surface_tension(p, cphi1, cphi2, cmuF1, cmuF2)         # :104
+    if p.gravity:
+        cnlz = cnlz + gravity_term(p, cphi1, cphi2)                      # :132-170
     cnlz = cnlz + forcing_spectrum(p)
     cux, cuy = get_velocity(p, cwz)                                      # :258-260
     tux = ifft(p, truncation(p, cux))                                    # :262-264
     tuy = ifft(p, truncation(p, cuy))
     tphi1 = ifft(p, truncation(p, _i_times(kx[:, None], cphi1)))         # :272-284
     tphi2 = ifft(p, truncation(p, _i_times(ky[None, :], cphi1)))
-    cnlphi1 = dfft(tphi1 * tux + tphi2 * tuy)                            # :292-300
+    wall = (1.0 - vpm_fields(p)[0])[:, None] if p.vpm else 1.0
+    cnlphi1 = dfft((tphi1 * tux + tphi2 * tuy) * wall)                   # :292-300
     tphi1 = ifft(p, truncation(p, _i_times(kx[:, None], cphi2)))         # :314-326
     tphi2 = ifft(p, truncation(p, _i_times(ky[None, :], cphi2)))
-    cnlphi2 = dfft(tphi1 * tux + tphi2 * tuy)                            # :334-348
+    cnlphi2 = dfft((tphi1 * tux + tphi2 * tuy) * wall)                   # :334-348
     cnlphi1 = -cnlphi1 - _cscale(cmuF1, (p.lam / g1) * ks)              # :369
     cnlphi2 = -cnlphi2 - _cscale(cmuF2, (p.lam / g2) * ks)              # :370
     return cnlz, cnlphi1, cnlphi2

@@ -53,6 +53,9 @@ class CParams(ctypes.Structure):
         ("Xlambda", ctypes.c_double),
         ("smooth_filter", ctypes.c_int),
         ("f_amp", ctypes.c_double), ("f_k", ctypes.c_int),
+        ("gravity", ctypes.c_int),
+        ("rho1", ctypes.c_double), ("rho2", ctypes.c_double), ("rho3", ctypes.c_double), ("g", ctypes.c_double),
+        ("vpm", ctypes.c_int), ("sponge", ctypes.c_int),
     ]
 
 

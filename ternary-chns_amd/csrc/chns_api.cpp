@@ -102,6 +102,17 @@ void fill_kparams(chns_ctx* c) {
     k.A21 = (d.g3 / d.g1); k.A22 = (1.0 + (d.g3 / d.g2));         // surface_tension.c:195
     k.lg1 = p.lambda / d.g1; k.lg2 = p.lambda / d.g2;             // nonlin.c:369-370
   }
+  k.grav = p.gravity ? 1 : 0;
+  k.vpm = p.vpm ? 1 : 0;
+  k.sponge = p.sponge ? 1 : 0;
+  if (p.model == CHNS_MODEL_TERNARY) {
+    k.gr1 = (-1.0 * p.g) * (p.rho1 - p.rho3);                     // nonlin.c:151-154
+    k.gr2 = (-1.0 * p.g) * (p.rho2 - p.rho3);
+  } else {
+    k.gr1 = (-1.0 * p.g) * (p.rho1 - p.rho2);                     // nonlin.c:117-118
+    k.gr2 = 0.0;
+  }
+  k.etainv = 1.0 / (1.5 * p.dt);                                  // global.h:48-49
   k.mcb = 1.5 * p.sigma * eps;                                    // surface_tension.c:141
   k.s24 = 24.0 * (p.sigma / eps);                                 // surface_tension.c:142, nonlin.c:240
   k.lam = p.lambda;                                               // nonlin.c:251
@@ -219,6 +230,19 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
       (e = hipMemcpy(dc2, tc2.data(), tc2.size() * 8, hipMemcpyHostToDevice)) != hipSuccess ||
       (e = hipMemcpy(dam, c->amask.data(), H * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
     return bail(hip_fail(c, e, "hipMemcpy(tables)"));
+  if (p->vpm) {
+    std::vector<double> chi(N), spg(N);
+    chns_host::build_wall_tables(p, chi.data(), spg.data());
+    double *dchi, *dspg;
+    if ((rc = dalloc(c, &dchi, N)) || (rc = dalloc(c, &dspg, N))) return bail(rc);
+    if ((e = hipMemcpy(dchi, chi.data(), N * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(dspg, spg.data(), N * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(hip_fail(c, e, "hipMemcpy(walls)"));
+    b.chi = dchi;
+    b.spg = dspg;
+  }
+  if (p->gravity && (rc = dalloc(c, &b.GR, S))) return bail(rc);
+  if (p->sponge && (rc = dalloc(c, &b.Hw, RJ))) return bail(rc);
   b.tw = dtw;
   b.tabw1 = dw1;
   b.tabc1 = dc1;
@@ -316,6 +340,8 @@ extern "C" int chns_step(chns_ctx* c, int nsteps) {
       if (c->nph > 0) CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] { return chns::launch_rowB(L, M, B, K, s); }));
       CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] { return chns::launch_colC(L, M, B, K, s); }));
       CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] { return chns::launch_rowD(L, M, B, K, s); }));
+      if (K.grav) CHNS_LAUNCH(c, chns::launch_gravity(B.C1, c->nph == 2 ? B.C2 : nullptr, B.GR, K.gr1, K.gr2, K.f,
+                                                      c->d.N, s));
       CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2,
                            [&] { return chns::launch_colE(L, M, st, B, K, 1 + c->nph, s); }));
     }

@@ -53,6 +53,11 @@ int launch_truncate(cd* S, const int* amask, int N, hipStream_t s) {
   k_truncate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(S, amask, N, N / 2 + 1);
   return (int)hipGetLastError();
 }
+int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, double f, int N, hipStream_t s) {
+  const size_t n = (size_t)N * (N / 2 + 1);
+  k_gravity<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(C1, C2, GR, gr1, gr2, f, N);
+  return (int)hipGetLastError();
+}
 int col_tile(int log2n) { CHNS_DISPATCH(log2n, (ColCfg<L>::C)); }
 
 namespace {

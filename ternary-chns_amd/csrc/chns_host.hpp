@@ -19,5 +19,7 @@ void derive(const chns_params* p, Derived* d);
 void build_coef_tables(const chns_params* p, double* tw1, double* tc1, double* tw2, double* tc2);
 void build_twiddles(int N, double* tw);
 void dealias_extent(const chns_params* p, int* amax);
+// wall profile chi[Nx] and sponge[Ny] (times 1/eta) of init.c:184-211
+void build_wall_tables(const chns_params* p, double* chi, double* sponge);
 
 }  // namespace chns_host

@@ -370,6 +370,13 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
     col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
+    if (P.sponge) {
+      // T(omega) itself for the sponge term (nonlin.c:50-54)
+#pragma unroll
+      for (int m = 0; m < CF::P; ++m) v[m] = s[m];
+      fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+      col_store_rm<LOG2N, CF::P, TJ>(B.Hw, P.ldJ, j, ok, t, v);
+    }
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
@@ -420,11 +427,39 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+    if (P.vpm) {
+      // nlz, then -u chi/eta, -v chi/eta whose curl is subtracted (nonlin.c:62-95);
+      // + omega sponge(y)/eta with the sponge (nonlin.c:50-54,68-70,97-99)
+      double nl[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
-    cd* o = B.G1 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
+      for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
+      if (P.sponge) {
+        row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
+        rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) nl[m] += (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)];
+      }
+      const double ce = B.chi[x] * P.etainv;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(-nl[m], -u[m] * ce);
+      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      cd* o = B.G1 + fo;
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+        const cd iky = ctimes_ik(Bv, P.f * (double)n);
+        o[TF * n] = mk(A.x - iky.x, A.y - iky.y);
+      });
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(-w[m] * ce, 0.0);
+      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      cd* o2 = B.G2 + fo;
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o2[TF * n] = A; });
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
+      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      cd* o = B.G1 + fo;
+      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
+    }
   } else {
     // Order: mu -> A_i, then velocity, then grad c_i (surface-tension force
     // F = sum_i A_i grad c_i, surface_tension.c:138-220, and advection u.grad c_i,
@@ -511,6 +546,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         }
       }
     }
+    // walls: F and u.grad c_i times (1 - chi), F minus u chi/eta (surface_tension.c:
+    // 83-85,151-154,202-205; nonlin.c:63-66,209-211,294-296,337-339)
+    const double om = P.vpm ? 1.0 - B.chi[x] : 1.0;
+    const double ce = P.vpm ? B.chi[x] * P.etainv : 0.0;
+    if (P.vpm) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        fx[m] = fx[m] * om - u[m] * ce;
+        fy[m] = fy[m] * om - w[m] * ce;
+        ad1[m] *= om;
+        if (TER) ad2[m] *= om;
+      }
+    }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
     rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
@@ -527,8 +575,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     }
     rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+    if (P.sponge) {
+      // + omega sponge(y)/eta in nlz (nonlin.c:50-54,68-70,97-99)
+      double nl[8];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
+      for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
+      row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
+      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(-(nl[m] + (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)]), fx[m]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
+    }
     rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
     cd* o1 = B.G1 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
@@ -567,12 +626,18 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
   col_load_rm<LOG2N, CF::P, TF>(Gm, P.ldF, j, ok, t, v);
   fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
   if (g == 0) {
-    if constexpr (MODEL != MODEL_NS) {
-      // + i kx F(Fy): the curl of the surface-tension force (curl.c:26)
+    if (MODEL != MODEL_NS || P.vpm) {
+      // + i kx F(Fy): the curl of the surface-tension force (curl.c:26) and of
+      // the wall penalisation (nonlin.c:76-80)
       col_load_rm<LOG2N, CF::P, TF>(B.G2, P.ldF, j, ok, t, s);
       fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
+    }
+    if (P.grav && ok) {
+      // buoyancy torque (nonlin.c:107-171), from k_gravity
+#pragma unroll
+      for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], B.GR[(size_t)j * N + t + m * T]);
     }
     if (P.f_k > 0 && j == P.f_k && t == 0) v[0].y += P.force_im;   // Kolmogorov forcing at (0, f_k)
   } else {
@@ -682,6 +747,22 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* _
       out[(size_t)j * N + i] = (absidx<LOG2N>(i) <= a) ? v[m] : mk(0.0, 0.0);
     }
   }
+}
+
+// Buoyancy torque i kx F(-g (rho(c) - 1)) of nonlin.c:107-171.  rho is linear
+// in the untruncated c, so the transform pair of the reference reduces to
+// GR = i kx (gr1 c1 + gr2 c2) on the spectral state (the constant rho3 - 1
+// only reaches kx = 0).  Runs before pass E, which updates the state in place.
+// grid: ceil(Ny2*N / 256)
+static __global__ void k_gravity(const cd* __restrict__ C1, const cd* __restrict__ C2, cd* GR, double gr1,
+                                 double gr2, double f, int N) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)N * (N / 2 + 1)) return;
+  const int i = (int)(e % N);
+  const double kx = f * (double)(i <= N / 2 ? i : i - N);
+  cd r = cscale(C1[e], gr1);
+  if (C2) r = cadd(r, cscale(C2[e], gr2));
+  GR[e] = ctimes_ik(r, kx);
 }
 
 // In-place truncation of a column-major spectral field (spectrum.c:24-26).

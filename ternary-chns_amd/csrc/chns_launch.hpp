@@ -18,6 +18,7 @@ int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, 
 int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF,
                    hipStream_t s);
 int launch_truncate(cd* S, const int* amask, int N, hipStream_t s);
+int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, double f, int N, hipStream_t s);
 int col_tile(int log2n);   // columns per column-kernel workgroup
 
 // batched passes (chns_batch.hip): nb instances of one size and model, device

@@ -38,6 +38,18 @@ extern "C" void chns_params_default(chns_params* p, int model) {
   p->smooth_filter = 0;
   p->f_amp = 0.0;
   p->f_k = 0;
+  p->gravity = 0;
+  if (model == CHNS_MODEL_BINARY) {
+    p->rho1 = 1.044;          // global.h:108-110
+    p->rho2 = 1.0;
+  } else {
+    p->rho1 = 0.957;          // global.h:137-140
+    p->rho2 = 1.044;
+  }
+  p->rho3 = 1.0;
+  p->g = 3.0;
+  p->vpm = 0;
+  p->sponge = 0;
 }
 
 namespace chns_host {
@@ -59,12 +71,34 @@ const char* check_params(const chns_params* p) {
   if (!(p->dt > 0.0)) return "dt must be positive";
   if (p->smooth_filter) return "SMOOTH_FILTER is not supported by this build";
   if (p->f_amp != 0.0 && !(p->f_k > 0 && p->f_k < p->Ny / 2)) return "forcing f_k must lie in (0, Ny/2)";
+  if (p->gravity && p->model == CHNS_MODEL_NS) return "gravity needs a phase field (BINARY or TERNARY)";
+  if (p->sponge && !p->vpm) return "sponge needs vpm (its profile is set up with the walls, init.c:184-211)";
+  if ((p->vpm || p->sponge) && !(p->dt > 0.0)) return "vpm needs dt > 0 (eta = 1.5 dt)";
   if (p->model == CHNS_MODEL_TERNARY) {
     Derived d;
     derive(p, &d);
     if (d.g1 == 0.0 || d.g2 == 0.0 || d.g3 == 0.0) return "ternary gammas must be non-zero";
   }
   return nullptr;
+}
+
+// chi(x_i) and sponge(y_j) (the latter times 1/eta) of init.c:184-211:
+// eps = 3 dx, walls at 6 dx and l - 6 dx, eta = 1.5 dt (global.h:48).
+void build_wall_tables(const chns_params* p, double* chi, double* sponge) {
+  const double dx = p->lx / p->Nx, dy = p->ly / p->Ny;
+  const double eps = 3.0 * dx;
+  const double yd = 6.0 * dy, xd = 6.0 * dx;
+  const double yu = p->ly - yd, xu = p->lx - xd;
+  const double eta_inv = 1.0 / (1.5 * p->dt);
+  for (int i = 0; i < p->Nx; ++i) {
+    const double x = i * dx;
+    chi[i] = 0.5 * (1.0 - tanh(2.0 * (x - xd) / eps)) + 0.5 * (1.0 + tanh(2.0 * (x - xu) / eps));
+  }
+  for (int j = 0; j < p->Ny; ++j) {
+    const double y = j * dy;
+    sponge[j] = 0.5 * (1.0 - tanh(2.0 * (y - yd) / eps)) + 0.5 * (1.0 + tanh(2.0 * (y - yu) / eps));
+    sponge[j] *= eta_inv;
+  }
 }
 
 void derive(const chns_params* p, Derived* d) {

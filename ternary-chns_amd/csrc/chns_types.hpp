@@ -24,6 +24,10 @@ struct KParams {
   double mcb;                  // 1.5*sigma*epsilon
   double s24;                  // 24*(sigma/epsilon)
   double lam;
+  // optional physics (macro.h GRAVITY, VPM, SPONGE)
+  int grav, vpm, sponge;
+  double gr1, gr2;             // -g (rho1 - rho3), -g (rho2 - rho3); binary -g (rho1 - rho2), 0
+  double etainv;               // 1/eta, eta = 1.5 dt
 };
 
 struct Bufs {
@@ -43,6 +47,10 @@ struct Bufs {
   const double* tabw2;         // omega stage 2: z2
   const double* tabc2;         // phase stage 2: z2
   const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
+  const double* chi;           // [N] wall profile chi(x) (vpm)
+  const double* spg;           // [N] sponge(y)/eta (sponge)
+  cd* GR;                      // buoyancy torque, column-major (gravity)
+  cd* Hw;                      // column iFFT of T(omega), TJ tiled (sponge)
 };
 
 // ---- diagnostics (chns_diag.hpp)

@@ -18,6 +18,7 @@
  * Parameters are key=value arguments (names of global.h / macro.h):
  *   model=ns|binary|ternary Nx lx nu dt maxiter nfile sigma epsilon lambda
  *   sigma_12 sigma_13 sigma_23 Xlambda f_amp f_k
+ *   gravity rho1 rho2 rho3 g vpm sponge   (macro.h GRAVITY / VPM / SPONGE switches)
  *   init=noise|read|droplet|zero  nranks (noise: srand(10+2*rank) per slab)
  *   read1=path read2=path (raw float64 [Nx][Ny], INIT_READ_DATA; defaults
  *   init_lens/phi1.in, init_lens/phi2.in, binary init_field/phi.in)
@@ -94,6 +95,13 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "Xlambda")) p.Xlambda = atof(v);
     else if (!strcmp(k, "f_amp")) p.f_amp = atof(v);
     else if (!strcmp(k, "f_k")) p.f_k = atoi(v);
+    else if (!strcmp(k, "gravity")) p.gravity = atoi(v);   /* macro.h GRAVITY */
+    else if (!strcmp(k, "rho1")) p.rho1 = atof(v);
+    else if (!strcmp(k, "rho2")) p.rho2 = atof(v);
+    else if (!strcmp(k, "rho3")) p.rho3 = atof(v);
+    else if (!strcmp(k, "g")) p.g = atof(v);
+    else if (!strcmp(k, "vpm")) p.vpm = atoi(v);           /* macro.h VPM */
+    else if (!strcmp(k, "sponge")) p.sponge = atoi(v);     /* macro.h SPONGE */
     else if (!strcmp(k, "init")) init = v;
     else if (!strcmp(k, "nranks")) nranks = atoi(v);
     else if (!strcmp(k, "read1")) read1 = v;
