@@ -151,7 +151,6 @@ extern "C" int chns_spectrum(chns_ctx* c, int truncate_state, double* ek, double
   if (!c->dspec && ((rc = chns_dalloc_bytes(c, (void**)&c->dspec, npart * 8)) ||
                     (rc = chns_dalloc_bytes(c, (void**)&c->dspec_out, nout * 8))))
     return rc;
-  DHIP(c, hipMemsetAsync(c->dspec, 0, npart * 8, c->stream));
   chns::SpecArgs A{};
   A.W = c->b.W;
   A.C1 = c->b.C1;
@@ -167,7 +166,7 @@ extern "C" int chns_spectrum(chns_ctx* c, int truncate_state, double* ek, double
   A.nxy = nxy;
   A.nph = c->nph;
   A.part = c->dspec;
-  A.party = c->dspec + (size_t)5 * N * nsh;
+  A.party = c->dspec + (size_t)5 * N * nsh;   // shell-major partials, see k_spec_rows
   A.partx = A.party + (size_t)N * nxy;
   double* out = c->dspec_out;
   DLAUNCH(c, chns::launch_spectrum(A, out, out + 5 * nsh, out + 5 * nsh + nxy, c->stream));

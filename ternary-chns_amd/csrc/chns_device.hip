@@ -115,8 +115,7 @@ int launch_spectrum(const SpecArgs& A, double* out, double* outy, double* outx, 
   k_spec_rows<<<(A.N + 63) / 64, 64, 0, s>>>(A);
   int e = (int)hipGetLastError();
   if (e) return e;
-  const int n = A.nsh > A.nxy ? A.nsh : A.nxy;
-  k_spec_sum<<<(n + 63) / 64, 64, 0, s>>>(A, out, outy, outx);
+  k_spec_sum<<<A.nsh + A.nxy, 256, 0, s>>>(A, out, outy, outx);
   return (int)hipGetLastError();
 }
 

@@ -176,11 +176,16 @@ __global__ __launch_bounds__(RB_THREADS) void k_reduce_final(const double* __res
 
 // ---------------------------------------------------------------- spectrum
 // spectrum.c:52-121.  One thread per kx row i walks ky = 0..N/2; shells
-// ik = (int)(|k|/dk + 0.5) are non-decreasing along the row, so each row
-// writes every shell it touches once into rows of partial histograms
-// [N][nsh] (zeroed first); k_spec_sum then adds the rows in order.
-// Quantities: 0 ek, 1 ensk, 2 sph1 (sph), 3 sph2, 4 sp12; eky per row too;
-// ekx per row is one number (ikx fixed along the row).
+// ik = (int)(|k|/dk + 0.5) are non-decreasing along the row and step by at
+// most one, so row i covers every shell of [ik(i,0), ik(i,N/2)] once.  It
+// writes each shell's run sums into shell-major partials part[q][s][i]
+// (quantities 0 ek, 1 ensk, 2 sph1 (sph), 3 sph2, 4 sp12) and party[b][i]
+// (eky, b = iky); entries outside a row's range are never written, and the
+// reducer skips them, so nothing needs clearing.  ekx per row is one number.
+__device__ __forceinline__ int spec_shell(double kxsqr, double ky, double dk) {
+  const double ks = __dadd_rn(kxsqr, __dmul_rn(ky, ky));
+  return (int)((sqrt(ks) / dk) + 0.5);
+}
 
 __global__ void k_spec_rows(SpecArgs A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -194,7 +199,7 @@ __global__ void k_spec_rows(SpecArgs A) {
   double ex = 0.0;
   auto flush = [&](int s) {
     if (s >= 0 && s < A.nsh)
-      for (int q = 0; q < 5; ++q) A.part[((size_t)q * N + i) * A.nsh + s] = acc[q];
+      for (int q = 0; q < 5; ++q) A.part[((size_t)q * A.nsh + s) * N + i] = acc[q];
     for (int q = 0; q < 5; ++q) acc[q] = 0.0;
   };
   for (int j = 0; j < H; ++j) {
@@ -205,6 +210,9 @@ __global__ void k_spec_rows(SpecArgs A) {
     const int iky = (int)((sqrt(kysqr) / A.dk) + 0.5);
     if (ik != cur) {
       flush(cur);
+      // shells step by at most one along a row (|k| grows by at most dk per
+      // column); a skipped shell would still get its zero entries
+      for (int sk = cur + 1; cur >= 0 && sk < ik; ++sk) flush(sk);
       cur = ik;
     }
     const bool kept = ai <= A.amask[j];   // truncation of u and the phase fields (spectrum.c:22-27)
@@ -223,7 +231,7 @@ __global__ void k_spec_rows(SpecArgs A) {
     acc[0] += wh * eu;
     acc[1] += wh * ens;
     ex += wh * eu;
-    if (iky < A.nxy) A.party[(size_t)i * A.nxy + iky] = wh * eu;
+    if (iky < A.nxy) A.party[(size_t)iky * N + i] = wh * eu;
     if (A.nph >= 1 && kept) {
       const cd c1 = A.C1[idx];
       acc[2] += w2 * ((c1.x * c1.x + c1.y * c1.y) * A.scale2);
@@ -238,28 +246,44 @@ __global__ void k_spec_rows(SpecArgs A) {
   A.partx[i] = ex;
 }
 
-// out[q*nsh + s] = sum_i part[q][i][s]; outy[b] = sum_i party[i][b];
-// outx[b] = sum over rows with shell ikx(i) == b of partx[i] (i ascending).
+// Workgroup b < nsh: out[q*nsh + b] = sum over rows i (ascending, in a fixed
+// tree) of part[q][b][i] for the rows whose shell range holds b.  Workgroup
+// nsh + b: outy[b] = sum_i party[b][i] (the rows write every b <= N/2) and
+// outx[b] = sum over rows with ikx(i) == b of partx[i].  256 threads.
 __global__ void k_spec_sum(SpecArgs A, double* out, double* outy, double* outx) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = A.N;
-  if (s < A.nsh) {
-    for (int q = 0; q < 5; ++q) {
-      double acc = 0.0;
-      for (int i = 0; i < N; ++i) acc += A.part[((size_t)q * N + i) * A.nsh + s];
-      out[q * A.nsh + s] = acc;
+  __shared__ double red[5][256];
+  const int N = A.N, H = N / 2 + 1, tid = threadIdx.x;
+  double acc[5] = {0, 0, 0, 0, 0};
+  const bool shell = (int)blockIdx.x < A.nsh;
+  const int b = shell ? (int)blockIdx.x : (int)blockIdx.x - A.nsh;
+  for (int i = tid; i < N; i += 256) {
+    const double kx = A.f * (double)(i <= N / 2 ? i : i - N);
+    const double kxsqr = __dmul_rn(kx, kx);
+    if (shell) {
+      const int lo = spec_shell(kxsqr, 0.0, A.dk);
+      const int hi = spec_shell(kxsqr, A.f * (double)(H - 1), A.dk);
+      if (b >= lo && b <= hi)
+        for (int q = 0; q < 5; ++q) acc[q] += A.part[((size_t)q * A.nsh + b) * N + i];
+    } else {
+      if (b < H) acc[0] += A.party[(size_t)b * N + i];
+      const int ikx = (int)((sqrt(kxsqr) / A.dk) + 0.5);
+      if (ikx == b) acc[1] += A.partx[i];
     }
   }
-  if (s < A.nxy) {
-    double acc = 0.0, accx = 0.0;
-    for (int i = 0; i < N; ++i) {
-      acc += A.party[(size_t)i * A.nxy + s];
-      const double kx = A.f * (double)(i <= N / 2 ? i : i - N);
-      const int ikx = (int)((sqrt(__dmul_rn(kx, kx)) / A.dk) + 0.5);
-      if (ikx == s) accx += A.partx[i];
+  for (int q = 0; q < 5; ++q) red[q][tid] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w)
+      for (int q = 0; q < 5; ++q) red[q][tid] += red[q][tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (shell) {
+      for (int q = 0; q < 5; ++q) out[q * A.nsh + b] = red[q][0];
+    } else if (b < A.nxy) {
+      outy[b] = red[0][0];
+      outx[b] = red[1][0];
     }
-    outy[s] = acc;
-    outx[s] = accx;
   }
 }
 

@@ -72,8 +72,8 @@ struct SpecArgs {
   const int* amask;
   double f, dk, kymax, scale2;
   int N, nsh, nxy, nph;
-  double* part;    // [5][N][nsh]
-  double* party;   // [N][nxy]
+  double* part;    // [5][nsh][N]
+  double* party;   // [nxy][N]
   double* partx;   // [N]
 };
 
