@@ -200,6 +200,12 @@ int chns_adj_backward(chns_ctx *ctx, double *g_wz, double *g_phi1, double *g_phi
 int chns_adj_get(chns_ctx *ctx, double *wbar, double *c1bar, double *c2bar, double *partials);
 /* Free the adjoint's device storage (checkpoints, trajectory, work arrays). */
 int chns_adj_release(chns_ctx *ctx);
+/* store != 0 (default): the replay of a segment keeps the forward
+ * intermediates of its 2*ckpt_every stages (at 2048^2 about 236 MB each) when
+ * they fit in the free device memory, so the reverse sweep does not recompute
+ * them; 0: always recompute.  Takes effect at the next chns_adj_forward; the
+ * gradients agree to round-off (bitwise below 256^2). */
+int chns_adj_set_store(chns_ctx *ctx, int store);
 
 /* ---- host-only helpers (no device needed) ---------------------------- */
 

@@ -34,7 +34,7 @@ EXPORTS = (
     "chns_diag_sizes", "chns_ensemble_create", "chns_ensemble_destroy", "chns_ensemble_last_error",
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
-    "chns_adj_backward", "chns_adj_get", "chns_adj_release",
+    "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -117,6 +117,7 @@ def load_library(path: str = LIB_PATH):
     lib.chns_adj_backward.argtypes = [V, D, D, D, D]
     lib.chns_adj_get.argtypes = [V, D, D, D, D]
     lib.chns_adj_release.argtypes = [V]
+    lib.chns_adj_set_store.argtypes = [V, I]
     _lib = lib
     return lib
 
@@ -310,6 +311,10 @@ class Context:
         self._check(self.lib.chns_adj_get(self.h, *[_dp(o.view(np.float64)) for o in outs], _dp(part)),
                     "chns_adj_get")
         return outs, dict(zip(ADJ_PARTIALS, part.tolist()))
+
+    def adj_set_store(self, store: bool):
+        """Keep the forward intermediates of a replayed segment (default) or recompute them."""
+        self._check(self.lib.chns_adj_set_store(self.h, int(bool(store))), "chns_adj_set_store")
 
     def adj_release(self):
         self._check(self.lib.chns_adj_release(self.h), "chns_adj_release")

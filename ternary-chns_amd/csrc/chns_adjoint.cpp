@@ -30,6 +30,11 @@ struct chns_adj_state {
   double* dS = nullptr;          // [nshell]
   int* nbq = nullptr;            // [ADJ_NQ]
   int mb = 0, nsh = 0;
+  // forward intermediates of every stage of a replayed segment (store mode):
+  // slot 2k holds passes A-C of N(u_k), slot 2k+1 those of N(a_k)
+  int store = 1;                 // 1: keep them when device memory allows, 0: recompute
+  chns::cd* inter = nullptr;     // [2K][10 RJ + 2 RF]
+  int inter_K = 0;
 };
 
 namespace {
@@ -140,17 +145,40 @@ chns::Bufs bufs_at(chns_ctx* c, chns::cd* const* in, chns::cd* const* out) {
   return b;
 }
 
+// Forward intermediates of slot k (store mode): Hc1, Hc2, Hu, Hv, Hwx, Hwy,
+// Hmu1, Hmu2, Hdx1, Hdx2 ([N][ldJ] each), then Mh1, Mh2 ([N][ldF] each).
+size_t slot_entries(const chns_ctx* c) { return (size_t)c->d.N * (10 * (size_t)c->ldJ + 2 * (size_t)c->ldF); }
+void point_slot(chns_ctx* c, chns::Bufs& b, int k) {
+  const size_t RJ = (size_t)c->d.N * c->ldJ, RF = (size_t)c->d.N * c->ldF;
+  chns::cd* q = c->adj->inter + (size_t)k * slot_entries(c);
+  chns::cd** rj[] = {&b.Hc1, &b.Hc2, &b.Hu, &b.Hv, &b.Hwx, &b.Hwy, &b.Hmu1, &b.Hmu2, &b.Hdx1, &b.Hdx2};
+  for (int f = 0; f < 10; ++f) *rj[f] = q + f * RJ;
+  b.Mh1 = q + 10 * RJ;
+  b.Mh2 = q + 10 * RJ + RF;
+}
+
 // J_N(x)^T applied to the input left in Gi/Hxi/... by the previous adj_colE,
 // finished by adj_colE<stage>.  Partials go to slot (2 - stage).
-int vjp(chns_ctx* c, chns::cd* const* x, int stage) {
+// slot >= 0: passes A-C at x were kept in that intermediate slot by the
+// replay; otherwise they are recomputed here into the context's buffers.
+int vjp(chns_ctx* c, chns::cd* const* x, int stage, int slot) {
   const int L = c->d.log2n, M = CHNS_MODEL_TERNARY;
   hipStream_t s = c->stream;
-  const chns::Bufs b = bufs_at(c, x, nullptr);
-  ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
-  ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
-  ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
   const chns::cd* xc[3] = {x[0], x[1], x[2]};
-  const chns::AdjArgs A = adj_args(c, xc, 2 - stage);
+  chns::AdjArgs A = adj_args(c, xc, 2 - stage);
+  if (slot >= 0) {
+    chns::Bufs b = c->b;
+    point_slot(c, b, slot);
+    A.A.Hc1 = b.Hc1; A.A.Hc2 = b.Hc2; A.A.Hu = b.Hu; A.A.Hv = b.Hv; A.A.Hwx = b.Hwx; A.A.Hwy = b.Hwy;
+    A.A.Hmu1 = b.Hmu1; A.A.Hmu2 = b.Hmu2; A.A.Hdx1 = b.Hdx1; A.A.Hdx2 = b.Hdx2;
+    A.A.Mh1 = b.Mh1;
+    A.A.Mh2 = b.Mh2;
+  } else {
+    const chns::Bufs b = bufs_at(c, x, nullptr);
+    ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
+    ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
+    ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+  }
   ALAUNCH(c, chns::launch_adj_rowD(L, A, s));
   ALAUNCH(c, chns::launch_adj_colBC(L, A, s));
   ALAUNCH(c, chns::launch_adj_rowB(L, A, s));
@@ -158,15 +186,27 @@ int vjp(chns_ctx* c, chns::cd* const* x, int stage) {
   return 0;
 }
 
-// one forward ETD2RK stage of the replay: state in -> out (pass E out of place)
-int replay_stage(chns_ctx* c, chns::cd* const* in, chns::cd* const* out, int stage, bool need_colA) {
+// One forward ETD2RK stage of the replay: state in -> out (pass E out of
+// place).  slot >= 0: passes A-C write intermediate slot `slot` and pass E's
+// fused pass A writes slot next_slot (the next stage's input); full = false
+// stops after pass C (the intermediates of a stage whose output is unused).
+int replay_stage(chns_ctx* c, chns::cd* const* in, chns::cd* const* out, int stage, bool need_colA, int slot,
+                 int next_slot, bool full) {
   const int L = c->d.log2n, M = CHNS_MODEL_TERNARY;
   hipStream_t s = c->stream;
-  const chns::Bufs b = bufs_at(c, in, out);
+  chns::Bufs b = bufs_at(c, in, out);
+  if (slot >= 0) point_slot(c, b, slot);
   if (need_colA) ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
   ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
   ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+  if (!full) return 0;
   ALAUNCH(c, chns::launch_rowD(L, M, b, c->kp, s));
+  if (next_slot >= 0) {
+    chns::Bufs nb = c->b;
+    point_slot(c, nb, next_slot);
+    b.Hc1 = nb.Hc1;
+    b.Hc2 = nb.Hc2;
+  }
   ALAUNCH(c, chns::launch_colE(L, M, stage, b, c->kp, 3, s));
   return 0;
 }
@@ -224,6 +264,7 @@ void chns_adj_free(chns_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   afree(a->ckpt);
   afree(a->traj);
+  afree(a->inter);
   afree(a->ub);
   afree(a->ab);
   afree(a->gi);
@@ -235,6 +276,17 @@ void chns_adj_free(chns_ctx* c) {
   afree(a->nbq);
   delete a;
   c->adj = nullptr;
+}
+
+extern "C" int chns_adj_set_store(chns_ctx* c, int store) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->adj->store = store ? 1 : 0;
+  if (!store) {
+    afree(c->adj->inter);
+    c->adj->inter_K = 0;
+  }
+  return CHNS_OK;
 }
 
 extern "C" int chns_adj_release(chns_ctx* c) {
@@ -262,6 +314,22 @@ extern "C" int chns_adj_forward(chns_ctx* c, int nsteps, int ckpt_every) {
     a->traj_K = 0;
     if ((rc = amalloc(c, &a->traj, (size_t)(2 * K + 1) * 3 * a->S))) return rc;
     a->traj_K = K;
+  }
+  // intermediates of a segment: kept when they fit in 85 % of the free memory
+  if (a->store && a->inter_K < K) {
+    afree(a->inter);
+    a->inter_K = 0;
+    size_t fr = 0, tot = 0;
+    const size_t need = (size_t)2 * K * slot_entries(c) * sizeof(chns::cd);
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess && need < fr / 100 * 85) {
+      void* q = nullptr;
+      if (hipMalloc(&q, need) == hipSuccess) {
+        a->inter = (chns::cd*)q;
+        a->inter_K = K;
+      } else {
+        (void)hipGetLastError();   // fall back to recomputing passes A-C
+      }
+    }
   }
   a->nsteps = nsteps;
   a->K = K;
@@ -341,14 +409,19 @@ extern "C" int chns_adj_backward(chns_ctx* c, double* g_wz, double* g_phi1, doub
     chns::cd* u[3];
     chns::cd* av[3];
     chns::cd* un[3];
+    const bool st = a->inter && a->inter_K >= K;
     for (int k = 0; k < len; ++k) {
       for (int f = 0; f < 3; ++f) {
         u[f] = k == 0 ? field(a->ckpt, a->S, sg, f) : field(a->traj, a->S, k, f);
         av[f] = field(a->traj, a->S, K + 1 + k, f);
         un[f] = field(a->traj, a->S, k + 1, f);
       }
-      if ((rc = replay_stage(c, u, av, 1, k == 0))) return rc;
-      if (k + 1 < len && (rc = replay_stage(c, av, un, 2, false))) return rc;
+      const bool last = k + 1 == len;
+      if ((rc = replay_stage(c, u, av, 1, k == 0, st ? 2 * k : -1, st ? 2 * k + 1 : -1, true))) return rc;
+      // the last step's stage 2 only matters for its intermediates (store mode)
+      if ((!last || st) &&
+          (rc = replay_stage(c, av, un, 2, false, st ? 2 * k + 1 : -1, st && !last ? 2 * k + 2 : -1, !last)))
+        return rc;
     }
     // reverse sweep over the segment
     for (int k = len - 1; k >= 0; --k) {
@@ -356,7 +429,7 @@ extern "C" int chns_adj_backward(chns_ctx* c, double* g_wz, double* g_phi1, doub
         u[f] = k == 0 ? field(a->ckpt, a->S, sg, f) : field(a->traj, a->S, k, f);
         av[f] = field(a->traj, a->S, K + 1 + k, f);
       }
-      if ((rc = vjp(c, av, 2)) || (rc = vjp(c, u, 1))) return rc;
+      if ((rc = vjp(c, av, 2, st ? 2 * k + 1 : -1)) || (rc = vjp(c, u, 1, st ? 2 * k : -1))) return rc;
       ALAUNCH(c, chns::launch_adj_accum(a->part, 2, (size_t)chns::ADJ_NQ * a->mb, a->mb, a->nbq, a->acc, s));
     }
   }
