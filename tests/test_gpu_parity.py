@@ -205,3 +205,45 @@ def test_full_size_real_io():
     assert rel(r0, O.ifft(op, O.truncation(op, s0.cwz))) < 1e-13
     ref = O.free_energy(op, s0)
     assert all(abs(x - y) <= 1e-12 * abs(y) for x, y in zip(fe, ref))
+
+
+def test_config_binary_quench_128_1000_steps():
+    """Binary spinodal quench, 128^2, c0 = 0.5 + U[-0.1, 0.1] (SplitMix64 seed 1),
+    eps = 1.5 dx, sigma = 0.1, lambda = 1e-3, nu = 0.01, dt = 1e-3, 1000 steps: the
+    trajectory against the oracle (round-off grows through the coarsening, so the bar
+    after 1000 steps is 1e-9) and mass to 1e-13."""
+    N = 128
+    kw = dict(Nx=N, Ny=N, lx=2 * math.pi, ly=2 * math.pi, nu=0.01, dt=1e-3, lam=1e-3,
+              epsilon=1.5 * 2 * math.pi / N, sigma=0.1)
+    op = O.Params(model="binary", **kw)
+    (c0,) = O.init_noise_real(op, 1, 0.5, 0.1)
+    s0 = O.state_from_real(op, None, c0)
+    ref = O.run(op, s0, 1000)
+    with C.Context(C.params("binary", **kw)) as ctx:
+        ctx.set_real_state(None, c0)
+        ctx.step(1000)
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert rel(a, b) < 1e-9
+    assert abs(g[1][0, 0] - s0.cphi1[0, 0]) <= TOL_MASS * abs(s0.cphi1[0, 0])
+
+
+def test_config_ternary_forced_1024():
+    """Ternary 1024^2, c1, c2 = 1/3 + U[-0.05, 0.05] (seed 2), all sigma_ij = 0.1,
+    eps = 1.5 dx, lambda = 1e-3, nu = 5e-3, Xlambda = 2, Kolmogorov forcing 0.1 sin(4y):
+    10 steps against the oracle (dt = 5e-4 diverges at this size for the explicit
+    k^2 muF term; dt = 5e-4 (256/1024)^3 is used)."""
+    N = 1024
+    kw = dict(Nx=N, Ny=N, lx=2 * math.pi, ly=2 * math.pi, nu=5e-3, dt=5e-4 * (256.0 / N) ** 3, lam=1e-3,
+              epsilon=1.5 * 2 * math.pi / N, sigma_12=0.1, sigma_13=0.1, sigma_23=0.1, Xlambda=2.0,
+              f_amp=0.1, f_k=4)
+    op = O.Params(model="ternary", **kw)
+    c1, c2 = O.init_noise_real(op, 2, 1.0 / 3.0, 0.05)
+    s0 = O.state_from_real(op, None, c1, c2)
+    ref = O.run(op, s0, 10)
+    with C.Context(C.params("ternary", **kw)) as ctx:
+        ctx.set_real_state(None, c1, c2)
+        ctx.step(10)
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert rel(a, b) <= TOL_TRAJ
