@@ -145,7 +145,8 @@ def ensemble_members():
     within each size even b is binary (c = 1/2 + U[-0.05, 0.05]) and odd b ternary
     (c1, c2 = 1/3 + U[-0.05, 0.05]); sigma (each sigma_ij) ~ U[0.05, 0.15] from
     SplitMix64 seeded 1000 + b, the noise seeded 2000 + b; epsilon = 1.5 dx, lambda = 1e-3,
-    nu = 5e-3, dt = 5e-4 (256/N)^3, L = 2 pi, omega_0 = 0."""
+    nu = 5e-3, dt = 5e-4 (256/N)^3 (ternary: times (min |gamma_i| / 0.1)^2 when below 0.1),
+    L = 2 pi, omega_0 = 0."""
 
     from chns_amd import ic
 
@@ -157,8 +158,15 @@ def ensemble_members():
             L = 2.0 * math.pi
             sig = 0.05 + 0.1 * ic._splitmix64(1000 + b, 3)
             # dt = 5e-4 at 256^2; the explicit k^2 muF rate grows like N^3 at epsilon = 1.5 dx
-            # and 5e-4 diverges at 512^2 and 1024^2, so dt shrinks as (256/N)^3
-            kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=5e-4 * (256.0 / N) ** 3, lam=1e-3, epsilon=1.5 * L / N,
+            # and 5e-4 diverges at 512^2 and 1024^2, so dt shrinks as (256/N)^3; a ternary
+            # member's rate also grows as its smallest |gamma_i| shrinks (sigma ~ U[0.05, 0.15]
+            # gives gamma_i down to ~0), so its dt shrinks as (min |gamma_i| / 0.1)^2 below 0.1
+            dt = 5e-4 * (256.0 / N) ** 3
+            if model == "ternary":
+                s12, s13, s23 = (float(x) for x in sig)
+                gmin = min(abs(s12 - s23 + s13), abs(s12 - s13 + s23), abs(s23 - s12 + s13))
+                dt *= min(1.0, max(gmin, 1e-3) / 0.1) ** 2
+            kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=dt, lam=1e-3, epsilon=1.5 * L / N,
                       sigma=float(sig[0]), sigma_12=float(sig[0]), sigma_13=float(sig[1]), sigma_23=float(sig[2]))
             nf = 1 if model == "binary" else 2
             fields = ic.noise(N, N, 2000 + b, 0.5 if model == "binary" else 1.0 / 3.0, 0.05, nf)
@@ -217,7 +225,7 @@ def ensemble_bench(steps: int, warmup: int):
         "dtype": "f64",
         "data": "synthetic (SplitMix64 noise, seeds 1000+b / 2000+b)",
         "config": {"workload": "ensemble 32x256^2 + 16x512^2 + 4x1024^2, binary/ternary alternating, "
-                               "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3, one HIP graph replay per step",
+                               "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3 (ternary x (min|gamma|/0.1)^2), one HIP graph replay per step",
                    "instances": len(members), "groups": groups, "parallelism": "single GPU"},
         "sequential_s": seq,
         "batch_s": {"untuned": res[False][0], "tuned": res[True][0]},
