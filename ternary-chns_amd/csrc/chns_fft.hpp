@@ -104,10 +104,13 @@ template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw
   return INV ? mk(w.x, -w.y) : w;
 }
 
-// LDS index with one padding element after every 8: the Stockham writes at
-// stride R (>= 128 B) would otherwise hit the same banks.
-__device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
-template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + (1 << LOG2N) / 8; };
+// LDS index with one padding element after every 2^CHNS_LDS_PAD: the
+// Stockham writes at stride R (>= 128 B) would otherwise hit the same banks.
+#ifndef CHNS_LDS_PAD
+#define CHNS_LDS_PAD 3
+#endif
+__device__ __forceinline__ int lp(int i) { return i + (i >> CHNS_LDS_PAD); }
+template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + ((1 << LOG2N) >> CHNS_LDS_PAD); };
 
 // Table twiddles of one stage for thread t: w, w^2, w^4 (, w^8) of each of
 // its Q = P/R butterflies, w = exp(-+2 pi i k TWS/N), k = b mod NS.  They are
@@ -198,7 +201,17 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS
   }
 }
 
-__device__ __forceinline__ void lds_sync() { __syncthreads(); }
+// Workgroup barrier for LDS exchanges.  The fences order LDS accesses only
+// (this wave's LDS writes complete, lgkmcnt(0), before the barrier), so vector
+// memory loads in flight -- prefetched twiddles, rows staged by the DMA
+// stager, the next operand of a pass -- stay in flight across it;
+// __syncthreads() would also wait for every outstanding load (vmcnt(0)).
+// Only for exchanges through LDS: it makes no global memory visible.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 template <int LOG2N, int P, int S>
 __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
@@ -260,7 +273,7 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
 // transpose (decimation in time): it takes x in that layout and returns
 // X[t + 256 j].  So spectral data keeps the cyclic layout of fft() while
 // real-space points sit at w4_pos(t, j) (see below).
-constexpr int W4_Q = 576;   // LDS entries per wave quarter: lp(511) + 1
+constexpr int W4_Q = 512 + (512 >> CHNS_LDS_PAD);   // LDS entries per wave quarter: >= lp(511) + 1
 
 __device__ __forceinline__ int w4_pos(int t, int j) { return 4 * (t & 63) + (t >> 6) + 256 * j; }
 
@@ -269,11 +282,59 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// 512-point Stockham transform by one wave: lane l holds points l + 64 m
-// (cyclic in, cyclic out); lq is the wave's LDS quarter; twiddles are
-// W_512^e = tw[4 e] of the 2048-point table.
-template <bool INV, bool PRUNED_ZERO_UPPER>
-__device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const cd* __restrict__ tw, int l) {
+// Twiddles of the 4-wave transform for thread t: eight entries of the
+// 2048-point table, the inverse using their conjugates.
+//   e = 0, 1: W^t, W^(2t)               (cross-wave radix-4 stage)
+//   e = 2..4: W^(32k), W^(64k), W^(128k), k = l % 8   (wave stage NS = 8)
+//   e = 5..7: W^(4l), W^(8l), W^(16l), l = t % 64     (wave stage NS = 64)
+// HELD: loaded once (init) and kept in registers for every transform of the
+// kernel, so no table load queues behind the kernel's row loads and stores
+// (vector memory operations complete in issue order).  Otherwise each entry
+// is loaded from the table where the transform first needs it.
+template <bool HELD>
+struct W4Tw {
+  const cd* __restrict__ tw;
+  cd w[8];
+  static __device__ __forceinline__ int idx(int e, int t) {
+    const int l = t & 63, k = l & 7;
+    switch (e) {
+      case 0: return t;
+      case 1: return 2 * t;
+      case 2: return 32 * k;
+      case 3: return 64 * k;
+      case 4: return 128 * k;
+      case 5: return 4 * l;
+      case 6: return 8 * l;
+      default: return 16 * l;
+    }
+  }
+  __device__ __forceinline__ void init(const cd* __restrict__ table, int t) {
+    tw = table;
+    if constexpr (HELD) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) w[e] = tw[idx(e, t)];
+    }
+  }
+  template <bool INV>
+  __device__ __forceinline__ cd get(int e, int t) const {
+    if constexpr (HELD) {
+      // an opaque copy per use: products of twiddles (w^3, w^5, ...) are then
+      // formed inside each transform instead of being kept live across all
+      double x = w[e].x, y = w[e].y;
+      asm volatile("" : "+v"(x), "+v"(y));
+      return INV ? mk(x, -y) : mk(x, y);
+    } else {
+      return twid<INV>(tw, idx(e, t));
+    }
+  }
+};
+
+// 512-point Stockham transform by one wave: lane l = t % 64 holds points
+// l + 64 m (cyclic in, cyclic out); lq is the wave's LDS quarter; twiddles are
+// W_512^e = W^(4e) of the 2048-point table.
+template <bool INV, bool PRUNED_ZERO_UPPER, bool HELD>
+__device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const W4Tw<HELD>& W, int t) {
+  const int l = t & 63;
   // stage NS = 1 (no twiddles)
   {
     cd a[8];
@@ -281,7 +342,7 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const cd* __restrict_
     for (int m = 0; m < 8; ++m) a[m] = v[m];
     dft8<INV>(a);
     const int base = l * 8;
-    cd w1 = twid<INV>(tw, 4 * (l % 8) * 8), w2 = twid<INV>(tw, 8 * (l % 8) * 8), w4 = twid<INV>(tw, 16 * (l % 8) * 8);
+    cd w1 = W.template get<INV>(2, t), w2 = W.template get<INV>(3, t), w4 = W.template get<INV>(4, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) lq[lp(base + q)] = a[q];
     wave_sync();
@@ -301,9 +362,9 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const cd* __restrict_
     dft8<INV>(a);
     const int k = l % 8;
     const int b2 = (l / 8) * 64 + k;
-    w1 = twid<INV>(tw, 4 * l);
-    w2 = twid<INV>(tw, 8 * l);
-    w4 = twid<INV>(tw, 16 * l);
+    w1 = W.template get<INV>(5, t);
+    w2 = W.template get<INV>(6, t);
+    w4 = W.template get<INV>(7, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) lq[lp(b2 + q * 8)] = a[q];
     wave_sync();
@@ -324,12 +385,12 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const cd* __restrict_
 }
 
 // Inverse 2048-point transform (see above).  PRUNED: X[k] = 0 for 512 < k < 1536.
-template <bool PRUNED>
-__device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+template <bool PRUNED, bool HELD>
+__device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const W4Tw<HELD>& W, int t) {
   asm volatile("" : "+v"(t));
   lds_sync();
   const cd c1 = mk(kR, kR);   // W_2048^(-256) = exp(+i pi/4)
-  const cd w1t = twid<true>(tw, t), w2t = twid<true>(tw, 2 * t);
+  const cd w1t = W.template get<true>(0, t), w2t = W.template get<true>(1, t);
   if constexpr (PRUNED) {
     v[3] = mk(0.0, 0.0);
     v[4] = mk(0.0, 0.0);
@@ -358,20 +419,21 @@ __device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const cd* __restrict_
 #pragma unroll
   for (int j = 0; j < 8; ++j) v[j] = lq[lp(l + 64 * j)];
   wave_sync();
-  fft512_wave<true, false>(v, lq, tw, l);
+  fft512_wave<true, false>(v, lq, W, t);
 }
 
 // Forward 2048-point transform: x in the w4_pos layout -> X[t + 256 j].
-__device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
+template <bool HELD>
+__device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const W4Tw<HELD>& W, int t) {
   asm volatile("" : "+v"(t));
   lds_sync();
   const int q = t >> 6, l = t & 63;
   cd* lq = lds + q * W4_Q;
-  fft512_wave<false, false>(v, lq, tw, l);
+  fft512_wave<false, false>(v, lq, W, t);
   wave_sync();
 #pragma unroll
   for (int j = 0; j < 8; ++j) lq[lp(l + 64 * j)] = v[j];
-  const cd w1t = twid<false>(tw, t), w2t = twid<false>(tw, 2 * t);
+  const cd w1t = W.template get<false>(0, t), w2t = W.template get<false>(1, t);
   const cd c1 = mk(kR, -kR);   // W_2048^256 = exp(-i pi/4)
   lds_sync();
 #pragma unroll
@@ -403,10 +465,31 @@ __device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const cd* __restrict_
 template <int LOG2N, bool INV, int R, bool PRUNED = false>
 __device__ __forceinline__ void rfft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   if constexpr (LOG2N == 11 && R == 1) {
-    if constexpr (INV) fft_w4_inv<PRUNED>(v, lds, tw, t);
-    else fft_w4_fwd(v, lds, tw, t);
+    W4Tw<false> W;
+    W.init(tw, t);
+    if constexpr (INV) fft_w4_inv<PRUNED>(v, lds, W, t);
+    else fft_w4_fwd(v, lds, W, t);
   } else {
     fft<LOG2N, 8, INV, R, PRUNED>(v, lds, tw, t);
+  }
+}
+
+// Twiddles of a row kernel whose transforms share them: held in registers for
+// the 4-wave transform (init loads them; call it before the kernel's row
+// loads), the table otherwise.
+template <int LOG2N, int R>
+struct RowTw {
+  static constexpr bool W4 = LOG2N == 11 && R == 1;
+  W4Tw<W4> w;
+  __device__ __forceinline__ void init(const cd* __restrict__ tw, int t) { w.init(tw, t); }
+};
+template <int LOG2N, bool INV, int R, bool PRUNED = false>
+__device__ __forceinline__ void rfft(cd* v, cd* lds, const RowTw<LOG2N, R>& T, int t) {
+  if constexpr (RowTw<LOG2N, R>::W4) {
+    if constexpr (INV) fft_w4_inv<PRUNED>(v, lds, T.w, t);
+    else fft_w4_fwd(v, lds, T.w, t);
+  } else {
+    fft<LOG2N, 8, INV, R, PRUNED>(v, lds, T.w.tw, t);
   }
 }
 template <int LOG2N, int R>

@@ -223,13 +223,18 @@ struct RowStager {
 
 // Split the forward transform Z of a + ib into A = F(a), B = F(b) for
 // n = t + m*T <= N/2 and hand them to op(m, n, A, B).  Uses lds (LdsSize entries).
-template <int LOG2N, int S, class Op>
-__device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
-  constexpr int N = 1 << LOG2N;
-  constexpr int T = N / 8;
+// In two halves around the barrier: row_split_put writes Z to lds,
+// row_split_take (after a workgroup barrier) reads the pairs.
+template <int LOG2N, int S>
+__device__ __forceinline__ void row_split_put(const cd* v, cd* lds, int t) {
+  constexpr int T = (1 << LOG2N) / 8;
 #pragma unroll
   for (int m = 0; m < 8; ++m) lds[lp(t + m * T) * S] = v[m];
-  lds_sync();
+}
+template <int LOG2N, int S, class Op>
+__device__ __forceinline__ void row_split_take(const cd* lds, int t, Op op) {
+  constexpr int N = 1 << LOG2N;
+  constexpr int T = N / 8;
 #pragma unroll
   for (int m = 0; m < 5; ++m) {
     const int n = t + m * T;
@@ -239,6 +244,12 @@ __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
       op(m, n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
     }
   }
+}
+template <int LOG2N, int S, class Op>
+__device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
+  row_split_put<LOG2N, S>(v, lds, t);
+  lds_sync();
+  row_split_take<LOG2N, S>(lds, t, op);
 }
 
 // ============================================================ pass A
@@ -275,10 +286,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;
   cd v[8];
+  RowTw<LOG2N, RC::R> tws;
+  tws.init(B.tw, t);
   const cd* h1 = B.Hc1 + tix<TJ>(x, 0, P.ldJ);
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix<TJ>(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -301,7 +314,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-  rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+  rfft<LOG2N, false, RC::R>(v, lds, tws, t);
   cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
   cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
   row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
@@ -418,15 +431,17 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
   const size_t fo = tix<TF>(x, 0, P.ldF);   // ... and at +TF*k
   const double sc = P.scale;
   cd v[8];
+  RowTw<LOG2N, RC::R> tws;
+  tws.init(B.tw, t);
   if constexpr (MODEL == MODEL_NS) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
     if (P.vpm) {
       // nlz, then -u chi/eta, -v chi/eta whose curl is subtracted (nonlin.c:62-95);
       // + omega sponge(y)/eta with the sponge (nonlin.c:50-54,68-70,97-99)
@@ -435,14 +450,14 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
       if (P.sponge) {
         row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
-        rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+        rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) nl[m] += (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)];
       }
       const double ce = B.chi[x] * P.etainv;
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-nl[m], -u[m] * ce);
-      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
       cd* o = B.G1 + fo;
       row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
         const cd iky = ctimes_ik(Bv, P.f * (double)n);
@@ -450,13 +465,13 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       });
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-w[m] * ce, 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
       cd* o2 = B.G2 + fo;
       row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o2[TF * n] = A; });
     } else {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
       cd* o = B.G1 + fo;
       row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
     }
@@ -485,7 +500,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -504,7 +519,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         u[m] = v[m].x * sc;
@@ -519,7 +534,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -536,7 +551,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
-        rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+        rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -561,34 +576,39 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
-    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    rfft<LOG2N, false, RC::R>(v, lds, tws, t);
     cd* o2 = B.G2 + fo;
     cd* o3 = B.G3 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+    const auto put23 = [&](int, int n, cd A, cd Bv) {
       o2[TF * n] = A;
       o3[TF * n] = Bv;
-    });
+    };
     if constexpr (STG) {
+      // the grad w pair is taken from the stager before the stores of this
+      // split are issued, so its wait (vmcnt(0)) does not wait for them
+      row_split_put<LOG2N, RC::R>(v, lds, t);
       st.wait();
       st.template assemble<false, true>(P.f, t, v);
+      row_split_take<LOG2N, RC::R>(lds, t, put23);
     } else {
+      row_split<LOG2N, RC::R>(v, lds, t, put23);
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     }
-    rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
     if (P.sponge) {
       // + omega sponge(y)/eta in nlz (nonlin.c:50-54,68-70,97-99)
       double nl[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
       row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
-      rfft<LOG2N, true, RC::R, true>(v, lds, B.tw, t);
+      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-(nl[m] + (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)]), fx[m]);
     } else {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
     }
-    rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+    rfft<LOG2N, false, RC::R>(v, lds, tws, t);
     cd* o1 = B.G1 + fo;
     row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
       const cd iky = ctimes_ik(Bv, P.f * (double)n);
@@ -597,7 +617,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, B.tw, t);
+      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
       cd* o4 = B.G4 + fo;
       row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[TF * n] = A; });
     }
