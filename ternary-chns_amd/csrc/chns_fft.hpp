@@ -201,12 +201,13 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS
   }
 }
 
-// Workgroup barrier for LDS exchanges.  The fences order LDS accesses only
-// (this wave's LDS writes complete, lgkmcnt(0), before the barrier), so vector
-// memory loads in flight -- prefetched twiddles, rows staged by the DMA
-// stager, the next operand of a pass -- stay in flight across it;
-// __syncthreads() would also wait for every outstanding load (vmcnt(0)).
-// Only for exchanges through LDS: it makes no global memory visible.
+// Workgroup barrier for exchanges through LDS.  The fences order LDS accesses
+// only (this wave's LDS accesses complete, lgkmcnt(0), before the barrier), so
+// vector memory operations in flight -- the next operand of a pass, rows the
+// DMA stager copies into its LDS buffers (issued as asm, which the compiler
+// does not track) -- stay in flight across it; __syncthreads() would wait for
+// every one of them (vmcnt(0)).  It makes no global memory and no DMA copy
+// visible: a DMA buffer is read only after waiting for its copies (RowStager).
 __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -308,11 +309,19 @@ struct W4Tw {
       default: return 16 * l;
     }
   }
+  // WAIT: wait for the loads here, before the kernel issues its row copies
+  // (DMA stager): later uses then carry no wait, which the compiler would
+  // otherwise place at the first transform, behind every copy in flight.
+  template <bool WAIT = false>
   __device__ __forceinline__ void init(const cd* __restrict__ table, int t) {
     tw = table;
     if constexpr (HELD) {
 #pragma unroll
       for (int e = 0; e < 8; ++e) w[e] = tw[idx(e, t)];
+      if constexpr (WAIT) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) asm volatile("" : "+v"(w[e].x), "+v"(w[e].y));
+      }
     }
   }
   template <bool INV>
@@ -481,7 +490,8 @@ template <int LOG2N, int R>
 struct RowTw {
   static constexpr bool W4 = LOG2N == 11 && R == 1;
   W4Tw<W4> w;
-  __device__ __forceinline__ void init(const cd* __restrict__ tw, int t) { w.init(tw, t); }
+  template <bool WAIT = false>
+  __device__ __forceinline__ void init(const cd* __restrict__ tw, int t) { w.template init<WAIT>(tw, t); }
 };
 template <int LOG2N, bool INV, int R, bool PRUNED = false>
 __device__ __forceinline__ void rfft(cd* v, cd* lds, const RowTw<LOG2N, R>& T, int t) {

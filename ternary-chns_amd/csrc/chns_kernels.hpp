@@ -185,10 +185,30 @@ __device__ __forceinline__ void row_load_pair(const cd* __restrict__ X, const cd
   assemble_pair<LOG2N, YIK>(X, Y, TJ, Jt, f, t, v);
 }
 
+// One DMA copy: 16 bytes from src of each active lane l straight into LDS at
+// byte address lds + 16 l (gfx950 global_load_lds_dwordx4; lds, from M0, is
+// wave-uniform).  Written as asm so that the compiler's wait insertion does
+// not see the copy: it would make every LDS read wait for all copies in
+// flight (vmcnt(0)), as it cannot tell the staging buffers apart.  Readers
+// wait for the copies themselves (RowStager); the compiler's own waits only
+// count fewer operations than are outstanding, so they wait at least as long
+// as they must.  M0 is restored.
+__device__ __forceinline__ void dma_copy16(const cd* src, unsigned lds) {
+  unsigned saved;
+  asm volatile(
+      "s_mov_b32 %0, m0\n\t"
+      "s_mov_b32 m0, %2\n\t"
+      "s_nop 0\n\t"
+      "global_load_lds_dwordx4 %1, off\n\t"
+      "s_mov_b32 m0, %0"
+      : "=&s"(saved)
+      : "v"(src), "s"(lds)
+      : "memory");
+}
+
 // Streams the dealiased rows of the next field pair into LDS while the
-// current pair is transformed (one row per workgroup, T threads): gfx950
-// global_load_lds_dwordx4 copies 16 bytes per lane straight into LDS, lane l
-// of a wave into slot base + l of the staging area (base from M0, uniform).
+// current pair is transformed (one row per workgroup, T threads): lane l of a
+// wave copies one 16-byte entry into slot base + l of the staging area.
 template <int LOG2N>
 struct RowStager {
   static constexpr int N = 1 << LOG2N, T = N / 8, JMAX = N / 4 + 1;
@@ -206,7 +226,7 @@ struct RowStager {
         if constexpr (HASY) {
           if (s >= Jt) src = Y + TJ * (s - Jt);
         }
-        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(stg + base), 16, 0, 0);
+        dma_copy16(src, (unsigned)(size_t)(__attribute__((address_space(3))) cd*)(stg + base));
       }
     }
   }
@@ -214,6 +234,25 @@ struct RowStager {
   __device__ __forceinline__ void wait() const {
     __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
+  }
+  // Loads one issue makes per wave at Jt = JMAX: ceil((nslot - 64 w) / T)
+  // for wave w, at least min_loads(nslot) (the last wave).
+  static constexpr int min_loads(int nslot) { return (nslot - T + 64 + T - 1) / T; }
+  static constexpr int vmcnt_imm(int n) { return (n & 15) | ((n >> 4) << 14) | 0x0F70; }
+  // Copies of this stager have landed, while those of the one other stager
+  // issued after it (HASY_NEWER) may still be in flight; then every wave's.
+  // Vector memory operations complete in issue order, so waiting until at
+  // most n are outstanding, n no more than the newer issue made, waits for
+  // every one issued before that issue (later loads or stores only make it
+  // wait longer).
+  template <bool HASY_NEWER>
+  __device__ __forceinline__ void wait_older() const {
+    if (Jt == JMAX) {
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm(min_loads(HASY_NEWER ? 2 * JMAX : JMAX)));
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0F70);
+    }
+    lds_sync();
   }
   template <bool YIK, bool HASY>
   __device__ __forceinline__ void assemble(double f, int t, cd* v) const {
@@ -432,7 +471,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
   const double sc = P.scale;
   cd v[8];
   RowTw<LOG2N, RC::R> tws;
-  tws.init(B.tw, t);
+  tws.template init<RC::R == 1>(B.tw, t);   // the stager follows at one row per workgroup
   if constexpr (MODEL == MODEL_NS) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
@@ -483,20 +522,23 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     // complete; -u.grad w and Fx share the next forward transform, so
     // W1 = -F(u.grad w) - i ky F(Fx) (curl.c:26, ky part) comes out of its split.
     constexpr bool TER = MODEL == MODEL_TERNARY;
-    // One row per workgroup (N = 2048): each field pair is staged in LDS
-    // while the previous pair is transformed.
+    // One row per workgroup (N = 2048): the field pairs are staged in LDS,
+    // alternately in two buffers, so pair k+2 streams in while pairs k and
+    // k+1 are transformed.
     constexpr bool STG = RC::R == 1;
-    __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
-    const RowStager<LOG2N> st{stg_all, P.Jt};
+    __shared__ cd stgA[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+    __shared__ cd stgB[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+    const RowStager<LOG2N> sa{stgA, P.Jt}, sb{stgB, P.Jt};
     double u[8], w[8], fx[8], fy[8], ad1[8], ad2[8];
     {
       double A1[8], A2[8];
       if constexpr (STG) {
-        st.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
-        st.wait();
-        st.template assemble<false, TER>(P.f, t, v);
+        sa.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
+        sb.template issue<true>(B.Hu + ro, B.Hv + ro);
+        sa.template wait_older<true>();
+        sa.template assemble<false, TER>(P.f, t, v);
         lds_sync();
-        st.template issue<true>(B.Hu + ro, B.Hv + ro);
+        sa.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
@@ -512,10 +554,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         }
       }
       if constexpr (STG) {
-        st.wait();
-        st.template assemble<false, true>(P.f, t, v);
+        sb.template wait_older<true>();
+        sb.template assemble<false, true>(P.f, t, v);
         lds_sync();
-        st.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+        if (TER) sb.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
+        else sb.template issue<true>(B.Hwx + ro, B.Hwy + ro);
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
@@ -526,11 +569,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         w[m] = v[m].y * sc;
       }
       if constexpr (STG) {
-        st.wait();
-        st.template assemble<true, true>(P.f, t, v);
-        lds_sync();
-        if (TER) st.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
-        else st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
+        sa.template wait_older<true>();
+        sa.template assemble<true, true>(P.f, t, v);
+        if (TER) {
+          lds_sync();
+          sa.template issue<true>(B.Hwx + ro, B.Hwy + ro);
+        }
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
@@ -544,10 +588,8 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       }
       if constexpr (TER) {
         if constexpr (STG) {
-          st.wait();
-          st.template assemble<true, true>(P.f, t, v);
-          lds_sync();
-          st.template issue<true>(B.Hwx + ro, B.Hwy + ro);
+          sb.template wait_older<true>();
+          sb.template assemble<true, true>(P.f, t, v);
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
@@ -586,9 +628,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     if constexpr (STG) {
       // the grad w pair is taken from the stager before the stores of this
       // split are issued, so its wait (vmcnt(0)) does not wait for them
+      const RowStager<LOG2N>& sw = TER ? sa : sb;   // the grad w pair
       row_split_put<LOG2N, RC::R>(v, lds, t);
-      st.wait();
-      st.template assemble<false, true>(P.f, t, v);
+      sw.wait();
+      sw.template assemble<false, true>(P.f, t, v);
       row_split_take<LOG2N, RC::R>(lds, t, put23);
     } else {
       row_split<LOG2N, RC::R>(v, lds, t, put23);
