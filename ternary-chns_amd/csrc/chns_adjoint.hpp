@@ -140,11 +140,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
     w[m] = v[m].y * sc;
     v[m] = mk(-xi[m] * u[m], -xi[m] * w[m]);
   }
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o2 = A.O[2] + fo;
     cd* o3 = A.O[3] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o2[TF * n] = a;
         o3[TF * n] = ctimes_ik(b, -P.f * (double)n);
@@ -153,21 +152,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * u[m], -e2[m] * u[m]);
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o10 = A.O[10] + fo;
     cd* o12 = A.O[12] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) { o10[TF * n] = a; o12[TF * n] = b; }
     });
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * w[m], -e2[m] * w[m]);
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o11 = A.O[11] + fo;
     cd* o13 = A.O[13] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o11[TF * n] = ctimes_ik(a, -P.f * (double)n);
         o13[TF * n] = ctimes_ik(b, -P.f * (double)n);
@@ -198,11 +195,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
     vb[m] += -e2[m] * (v[m].y * sc);
     v[m] = mk(ub[m], vb[m]);
   }
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o0 = A.O[0] + fo;
     cd* o1 = A.O[1] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o0[TF * n] = ctimes_ik(a, -P.f * (double)n);   // -i ky R(ubar)
         o1[TF * n] = b;                                 // R(vbar)
@@ -262,21 +258,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
     sq[3] += Ab2[m] * m2;
     v[m] = mk(xiy[m] * A1[m], xiy[m] * A2[m]);
   }
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o4 = A.O[4] + fo;
     cd* o7 = A.O[7] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) { o4[TF * n] = a; o7[TF * n] = b; }
     });
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A1[m], P.A11 * Ab1[m] + P.A21 * Ab2[m]);
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o5 = A.O[5] + fo;
     cd* o6 = A.O[6] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o5[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc1 * __dmul_rn(ky, ky)));
@@ -286,11 +280,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A2[m], P.A12 * Ab1[m] + P.A22 * Ab2[m]);
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  {
+    {
     cd* o8 = A.O[8] + fo;
     cd* o9 = A.O[9] + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o8[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc2 * __dmul_rn(ky, ky)));
@@ -422,11 +415,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
                       Gb3 * (P.g3 * fp3 + P.X6 * a1 * a1 * a2 * a2);
     v[m] = mk(d1 - d3, d2 - d3);
   }
-  rfft<LOG2N, false, RC::R>(v, lds, A.tw, t);
-  cd* o1 = A.Ocb1 + tix<TF>(x, 0, P.ldJ);
+    cd* o1 = A.Ocb1 + tix<TF>(x, 0, P.ldJ);
   cd* o2 = A.Ocb2 + tix<TF>(x, 0, P.ldJ);
   const int Jt = P.Jt;
-  row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+  rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
     if (n < Jt) { o1[TF * n] = a; o2[TF * n] = b; }
   });
   block_sum<7, RC::NT>(sq, (double*)lds_all, A.part + blockIdx.x, A.mb);

@@ -467,6 +467,68 @@ __device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const W4Tw<HELD>& W, 
   lds_sync();   // callers (row_split) write the LDS array next
 }
 
+// Split of a paired real transform: Z = F(a + i b) -> A = F(a), B = F(b) at n
+// from zn = Z[n] and zr = Z[N - n].
+__device__ __forceinline__ void split_pair(cd zn, cd zr, cd* A, cd* B) {
+  *A = mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5);
+  *B = mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5);
+}
+
+// Forward 2048-point transform of the w4_pos layout fused with the split of a
+// paired real transform: hands A = F(a), B = F(b) for every n <= N/2 to
+// op(n, A, B), nothing in v.  The final radix-4 stage gives thread t the
+// positions k = t and k' = 512 - t (thread 0: 0 and 256): the mirror
+// N - (k + 512 q) of each output it forms is then k' + 512 (3 - q) (thread 0:
+// its own), so no exchange through LDS is needed for the split.  Twiddles of
+// k': W^(512 - t) = -i conj(W^t), W^(1024 - 2t) = -conj(W^(2t)).
+template <bool HELD, class Op>
+__device__ __forceinline__ void fft_w4_fwd_split(cd* v, cd* lds, const W4Tw<HELD>& W, int t, Op op) {
+  asm volatile("" : "+v"(t));
+  lds_sync();
+  const int q = t >> 6, l = t & 63;
+  cd* lq = lds + q * W4_Q;
+  fft512_wave<false, false>(v, lq, W, t);
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) lq[lp(l + 64 * j)] = v[j];
+  const cd w1t = W.template get<false>(0, t), w2t = W.template get<false>(1, t);
+  lds_sync();
+  const bool t0 = t == 0;
+  const int k2 = t0 ? 256 : 512 - t;
+  cd x[4], y[4];
+  {
+    const cd w3 = cmulc(w1t, w2t);
+    x[0] = lds[lp(t)];
+    x[1] = cmulc(lds[W4_Q + lp(t)], w1t);
+    x[2] = cmulc(lds[2 * W4_Q + lp(t)], w2t);
+    x[3] = cmulc(lds[3 * W4_Q + lp(t)], w3);
+    dft4<false>(x[0], x[1], x[2], x[3]);   // X[t + 512 q]
+  }
+  {
+    const cd u1 = t0 ? mk(kR, -kR) : mk(-w1t.y, -w1t.x);     // W^256 = exp(-i pi/4)
+    const cd u2 = t0 ? mk(0.0, -1.0) : mk(-w2t.x, w2t.y);    // W^512 = -i
+    const cd u3 = cmulc(u1, u2);
+    y[0] = lds[lp(k2)];
+    y[1] = cmulc(lds[W4_Q + lp(k2)], u1);
+    y[2] = cmulc(lds[2 * W4_Q + lp(k2)], u2);
+    y[3] = cmulc(lds[3 * W4_Q + lp(k2)], u3);
+    dft4<false>(y[0], y[1], y[2], y[3]);   // X[k2 + 512 q]
+  }
+  cd A, B;
+  split_pair(x[0], t0 ? x[0] : y[3], &A, &B);
+  op(t, A, B);
+  split_pair(x[1], t0 ? x[3] : y[2], &A, &B);
+  op(t + 512, A, B);
+  split_pair(y[0], t0 ? y[3] : x[3], &A, &B);
+  op(k2, A, B);
+  split_pair(y[1], t0 ? y[2] : x[2], &A, &B);
+  op(k2 + 512, A, B);
+  if (t0) {
+    split_pair(x[2], x[2], &A, &B);
+    op(1024, A, B);
+  }
+}
+
 // Row transform of a kernel with R rows per workgroup (rows interleaved
 // across lanes, 8 points per thread): the 4-wave transform at N = 2048 with
 // one row per workgroup, fft() otherwise.  rpos gives the real-space index of

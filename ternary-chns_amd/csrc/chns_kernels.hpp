@@ -22,6 +22,8 @@
 //             ETD2RK update with host-built coefficients (etd2rk.c:81-212),
 //             then the next stage's pass A for the dealiased columns.
 #pragma once
+#include <type_traits>
+
 #include "chns_fft.hpp"
 
 namespace chns {
@@ -261,7 +263,7 @@ struct RowStager {
 };
 
 // Split the forward transform Z of a + ib into A = F(a), B = F(b) for
-// n = t + m*T <= N/2 and hand them to op(m, n, A, B).  Uses lds (LdsSize entries).
+// n = t + m*T <= N/2 and hand them to op(n, A, B).  Uses lds (LdsSize entries).
 // In two halves around the barrier: row_split_put writes Z to lds,
 // row_split_take (after a workgroup barrier) reads the pairs.
 template <int LOG2N, int S>
@@ -278,9 +280,9 @@ __device__ __forceinline__ void row_split_take(const cd* lds, int t, Op op) {
   for (int m = 0; m < 5; ++m) {
     const int n = t + m * T;
     if (n <= N / 2) {
-      const cd zn = lds[lp(n) * S];
-      const cd zr = lds[lp((N - n) & (N - 1)) * S];
-      op(m, n, mk((zn.x + zr.x) * 0.5, (zn.y - zr.y) * 0.5), mk((zn.y + zr.y) * 0.5, (zr.x - zn.x) * 0.5));
+      cd A, B;
+      split_pair(lds[lp(n) * S], lds[lp((N - n) & (N - 1)) * S], &A, &B);
+      op(n, A, B);
     }
   }
 }
@@ -289,6 +291,26 @@ __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
   row_split_put<LOG2N, S>(v, lds, t);
   lds_sync();
   row_split_take<LOG2N, S>(lds, t, op);
+}
+
+// Forward row transform of the pair a + ib in v (real-space layout of rfft)
+// and its split, op(n, A, B) for n <= N/2: fused into the last stage of the
+// 4-wave transform, the transform then row_split otherwise.  Tw: RowTw or the
+// twiddle table.
+template <int LOG2N, int R, class Tw, class Op>
+__device__ __forceinline__ void rfft_split(cd* v, cd* lds, const Tw& tws, int t, Op op) {
+  if constexpr (LOG2N == 11 && R == 1) {
+    if constexpr (std::is_same<Tw, RowTw<LOG2N, R>>::value) {
+      fft_w4_fwd_split(v, lds, tws.w, t, op);
+    } else {
+      W4Tw<false> W;
+      W.init(tws, t);
+      fft_w4_fwd_split(v, lds, W, t, op);
+    }
+  } else {
+    rfft<LOG2N, false, R>(v, lds, tws, t);
+    row_split<LOG2N, R>(v, lds, t, op);
+  }
 }
 
 // ============================================================ pass A
@@ -353,10 +375,9 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-  rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-  cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
+    cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
   cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
-  row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd a, cd b) {
+  rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
     o1[TF * n] = a;
     if (MODEL == MODEL_TERNARY) o2[TF * n] = b;
   });
@@ -496,23 +517,20 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       const double ce = B.chi[x] * P.etainv;
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-nl[m], -u[m] * ce);
-      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-      cd* o = B.G1 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+            cd* o = B.G1 + fo;
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd Bv) {
         const cd iky = ctimes_ik(Bv, P.f * (double)n);
         o[TF * n] = mk(A.x - iky.x, A.y - iky.y);
       });
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-w[m] * ce, 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-      cd* o2 = B.G2 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o2[TF * n] = A; });
+            cd* o2 = B.G2 + fo;
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o2[TF * n] = A; });
     } else {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-      cd* o = B.G1 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
+            cd* o = B.G1 + fo;
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
     }
   } else {
     // Order: mu -> A_i, then velocity, then grad c_i (surface-tension force
@@ -618,23 +636,21 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     }
 #pragma unroll
     for (int m = 0; m < 8; ++m) v[m] = mk(fy[m], ad1[m]);
-    rfft<LOG2N, false, RC::R>(v, lds, tws, t);
     cd* o2 = B.G2 + fo;
     cd* o3 = B.G3 + fo;
-    const auto put23 = [&](int, int n, cd A, cd Bv) {
+    const auto put23 = [&](int n, cd A, cd Bv) {
       o2[TF * n] = A;
       o3[TF * n] = Bv;
     };
     if constexpr (STG) {
-      // the grad w pair is taken from the stager before the stores of this
-      // split are issued, so its wait (vmcnt(0)) does not wait for them
-      const RowStager<LOG2N>& sw = TER ? sa : sb;   // the grad w pair
-      row_split_put<LOG2N, RC::R>(v, lds, t);
+      // wait for the grad w pair before the stores of this transform are
+      // issued, so that the wait (vmcnt(0)) is not also for them
+      const RowStager<LOG2N>& sw = TER ? sa : sb;
       sw.wait();
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
       sw.template assemble<false, true>(P.f, t, v);
-      row_split_take<LOG2N, RC::R>(lds, t, put23);
     } else {
-      row_split<LOG2N, RC::R>(v, lds, t, put23);
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     }
     rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
@@ -651,18 +667,16 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
     }
-    rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-    cd* o1 = B.G1 + fo;
-    row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+        cd* o1 = B.G1 + fo;
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd Bv) {
       const cd iky = ctimes_ik(Bv, P.f * (double)n);
       o1[TF * n] = mk(A.x - iky.x, A.y - iky.y);
     });
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
-      rfft<LOG2N, false, RC::R>(v, lds, tws, t);
-      cd* o4 = B.G4 + fo;
-      row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd) { o4[TF * n] = A; });
+            cd* o4 = B.G4 + fo;
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o4[TF * n] = A; });
     }
   }
 }
@@ -776,10 +790,9 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
     const size_t o = (size_t)x * N + rpos<LOG2N, RC::R>(t, m);
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
-  rfft<LOG2N, false, RC::R>(v, lds, tw, t);
-  cd* pa = oa + tix<TF>(x, 0, ldF);
+    cd* pa = oa + tix<TF>(x, 0, ldF);
   cd* pb = ob ? ob + tix<TF>(x, 0, ldF) : nullptr;
-  row_split<LOG2N, RC::R>(v, lds, t, [&](int, int n, cd A, cd Bv) {
+  rfft_split<LOG2N, RC::R>(v, lds, tw, t, [&](int n, cd A, cd Bv) {
     pa[TF * n] = A;
     if (pb) pb[TF * n] = Bv;
   });
