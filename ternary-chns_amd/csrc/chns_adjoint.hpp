@@ -140,7 +140,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
     w[m] = v[m].y * sc;
     v[m] = mk(-xi[m] * u[m], -xi[m] * w[m]);
   }
-    {
+  {
     cd* o2 = A.O[2] + fo;
     cd* o3 = A.O[3] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * u[m], -e2[m] * u[m]);
-    {
+  {
     cd* o10 = A.O[10] + fo;
     cd* o12 = A.O[12] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-e1[m] * w[m], -e2[m] * w[m]);
-    {
+  {
     cd* o11 = A.O[11] + fo;
     cd* o13 = A.O[13] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
     vb[m] += -e2[m] * (v[m].y * sc);
     v[m] = mk(ub[m], vb[m]);
   }
-    {
+  {
     cd* o0 = A.O[0] + fo;
     cd* o1 = A.O[1] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
     sq[3] += Ab2[m] * m2;
     v[m] = mk(xiy[m] * A1[m], xiy[m] * A2[m]);
   }
-    {
+  {
     cd* o4 = A.O[4] + fo;
     cd* o7 = A.O[7] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -267,7 +267,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A1[m], P.A11 * Ab1[m] + P.A21 * Ab2[m]);
-    {
+  {
     cd* o5 = A.O[5] + fo;
     cd* o6 = A.O[6] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   }
 #pragma unroll
   for (int m = 0; m < 8; ++m) v[m] = mk(-xix[m] * A2[m], P.A12 * Ab1[m] + P.A22 * Ab2[m]);
-    {
+  {
     cd* o8 = A.O[8] + fo;
     cd* o9 = A.O[9] + fo;
     rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {

@@ -394,10 +394,12 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const W4Tw<HELD>& W, 
 }
 
 // Inverse 2048-point transform (see above).  PRUNED: X[k] = 0 for 512 < k < 1536.
-template <bool PRUNED, bool HELD>
+// SYNC = false: the caller has synchronised the workgroup since any other
+// wave last accessed lds (the transform's first writes cross waves).
+template <bool PRUNED, bool HELD, bool SYNC = true>
 __device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const W4Tw<HELD>& W, int t) {
   asm volatile("" : "+v"(t));
-  lds_sync();
+  if constexpr (SYNC) lds_sync();
   const cd c1 = mk(kR, kR);   // W_2048^(-256) = exp(+i pi/4)
   const cd w1t = W.template get<true>(0, t), w2t = W.template get<true>(1, t);
   if constexpr (PRUNED) {
@@ -481,10 +483,12 @@ __device__ __forceinline__ void split_pair(cd zn, cd zr, cd* A, cd* B) {
 // N - (k + 512 q) of each output it forms is then k' + 512 (3 - q) (thread 0:
 // its own), so no exchange through LDS is needed for the split.  Twiddles of
 // k': W^(512 - t) = -i conj(W^t), W^(1024 - 2t) = -conj(W^(2t)).
-template <bool HELD, class Op>
+// SYNC = false: no other wave has accessed this wave's quarter of lds since
+// the caller's last workgroup barrier (the first stages stay in the quarter).
+template <bool HELD, bool SYNC = true, class Op>
 __device__ __forceinline__ void fft_w4_fwd_split(cd* v, cd* lds, const W4Tw<HELD>& W, int t, Op op) {
   asm volatile("" : "+v"(t));
-  lds_sync();
+  if constexpr (SYNC) lds_sync();
   const int q = t >> 6, l = t & 63;
   cd* lq = lds + q * W4_Q;
   fft512_wave<false, false>(v, lq, W, t);
@@ -555,10 +559,12 @@ struct RowTw {
   template <bool WAIT = false>
   __device__ __forceinline__ void init(const cd* __restrict__ tw, int t) { w.template init<WAIT>(tw, t); }
 };
-template <int LOG2N, bool INV, int R, bool PRUNED = false>
+// SYNC = false skips the opening barrier of the 4-wave inverse transform
+// (see fft_w4_inv); the other transforms always synchronise.
+template <int LOG2N, bool INV, int R, bool PRUNED = false, bool SYNC = true>
 __device__ __forceinline__ void rfft(cd* v, cd* lds, const RowTw<LOG2N, R>& T, int t) {
   if constexpr (RowTw<LOG2N, R>::W4) {
-    if constexpr (INV) fft_w4_inv<PRUNED>(v, lds, T.w, t);
+    if constexpr (INV) fft_w4_inv<PRUNED, true, SYNC>(v, lds, T.w, t);
     else fft_w4_fwd(v, lds, T.w, t);
   } else {
     fft<LOG2N, 8, INV, R, PRUNED>(v, lds, T.w.tw, t);

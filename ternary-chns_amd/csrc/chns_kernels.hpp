@@ -297,11 +297,13 @@ __device__ __forceinline__ void row_split(const cd* v, cd* lds, int t, Op op) {
 // and its split, op(n, A, B) for n <= N/2: fused into the last stage of the
 // 4-wave transform, the transform then row_split otherwise.  Tw: RowTw or the
 // twiddle table.
-template <int LOG2N, int R, class Tw, class Op>
+// SYNC = false skips the opening barrier of the fused 4-wave transform (see
+// fft_w4_fwd_split); the other transforms always synchronise.
+template <int LOG2N, int R, bool SYNC = true, class Tw, class Op>
 __device__ __forceinline__ void rfft_split(cd* v, cd* lds, const Tw& tws, int t, Op op) {
   if constexpr (LOG2N == 11 && R == 1) {
     if constexpr (std::is_same<Tw, RowTw<LOG2N, R>>::value) {
-      fft_w4_fwd_split(v, lds, tws.w, t, op);
+      fft_w4_fwd_split<true, SYNC>(v, lds, tws.w, t, op);
     } else {
       W4Tw<false> W;
       W.init(tws, t);
@@ -517,19 +519,19 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       const double ce = B.chi[x] * P.etainv;
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-nl[m], -u[m] * ce);
-            cd* o = B.G1 + fo;
+      cd* o = B.G1 + fo;
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd Bv) {
         const cd iky = ctimes_ik(Bv, P.f * (double)n);
         o[TF * n] = mk(A.x - iky.x, A.y - iky.y);
       });
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-w[m] * ce, 0.0);
-            cd* o2 = B.G2 + fo;
+      cd* o2 = B.G2 + fo;
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o2[TF * n] = A; });
     } else {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
-            cd* o = B.G1 + fo;
+      cd* o = B.G1 + fo;
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
     }
   } else {
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -580,7 +582,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         u[m] = v[m].x * sc;
@@ -596,7 +598,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -611,7 +613,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
-        rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+        rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       // issued, so that the wait (vmcnt(0)) is not also for them
       const RowStager<LOG2N>& sw = TER ? sa : sb;
       sw.wait();
-      rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
+      rfft_split<LOG2N, RC::R, false>(v, lds, tws, t, put23);   // after the wait's barrier
       sw.template assemble<false, true>(P.f, t, v);
     } else {
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
@@ -667,15 +669,17 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc)), fx[m]);
     }
-        cd* o1 = B.G1 + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd Bv) {
+    cd* o1 = B.G1 + fo;
+    // no other wave touches this wave's lds quarter after the grad w inverse
+    // transform's exchange (the sponge's is alike)
+    rfft_split<LOG2N, RC::R, !STG>(v, lds, tws, t, [&](int n, cd A, cd Bv) {
       const cd iky = ctimes_ik(Bv, P.f * (double)n);
       o1[TF * n] = mk(A.x - iky.x, A.y - iky.y);
     });
     if constexpr (TER) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(ad2[m], 0.0);
-            cd* o4 = B.G4 + fo;
+      cd* o4 = B.G4 + fo;
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o4[TF * n] = A; });
     }
   }
