@@ -47,7 +47,8 @@ typedef struct {
   double lx, ly, nu, dt;
   double sigma, epsilon, lambda;
   double sigma_12, sigma_13, sigma_23, Xlambda;
-  int smooth_filter; /* macro.h:15 SMOOTH_FILTER; must be 0 in this build */
+  int smooth_filter; /* macro.h:15 SMOOTH_FILTER: truncation() multiplies by exp(-36 pp^18) twice
+                        instead of cutting (functions.c:70-73); single contexts only */
   double f_amp;
   int f_k;
   int gravity;       /* macro.h GRAVITY */
@@ -218,6 +219,11 @@ int chns_etd_tables(const chns_params *p, double *eta, double *z1w, double *z2w,
 /* Dealias extent per ky column j < Ny/2+1: the largest |kx index| kept by
  * truncation() (functions.c:47-83), -1 if the column is fully truncated. */
 int chns_dealias_extent(const chns_params *p, int *amax);
+
+/* SMOOTH_FILTER factor exp(-36 pp^18), pp = ksqr/ksqrmax, per mode in IDXC
+ * order [Nx][Ny2] (functions.c:54,70-73, C library exp and pow); truncation()
+ * multiplies by it twice.  Defined whatever p->smooth_filter is. */
+int chns_filter_table(const chns_params *p, double *ft);
 
 #ifdef __cplusplus
 }

@@ -160,11 +160,24 @@ def dealias_mask(p: Params):
     return ~(ksqr_grid(p) > dealias)                   # functions.c:75-78
 
 
+_FILTER_CACHE = {}
+
+
 def smooth_filter_factor(p: Params):
-    """SMOOTH_FILTER multiplier exp(-36*pp^18), applied twice (functions.c:70-73)."""
-    ksqrmax = float(p.kxmax * p.kxmax + p.kymax * p.kymax)   # functions.c:54
-    pp = ksqr_grid(p) / ksqrmax
-    return np.array([math.exp(-36.0 * math.pow(v, 18)) for v in pp.ravel()]).reshape(pp.shape)
+    """SMOOTH_FILTER multiplier exp(-36*pp^18), applied twice (functions.c:70-73).
+
+    Evaluated with the C library's exp and pow (math.exp, math.pow) per mode,
+    cached per grid (read-only array).
+    """
+    key = (p.Nx, p.Ny, p.lx, p.ly)
+    f = _FILTER_CACHE.get(key)
+    if f is None:
+        ksqrmax = float(p.kxmax * p.kxmax + p.kymax * p.kymax)   # functions.c:54
+        pp = ksqr_grid(p) / ksqrmax
+        f = np.array([math.exp(-36.0 * math.pow(v, 18)) for v in pp.ravel()]).reshape(pp.shape)
+        f.setflags(write=False)
+        _FILTER_CACHE[key] = f
+    return f
 
 
 def truncation(p: Params, cv):

@@ -46,6 +46,7 @@ def build(verbose: bool = False) -> str:
         "ensemble": os.path.join(BUILD, "chns_ensemble.o"),
         "adjapi": os.path.join(BUILD, "chns_adjoint_api.o"),
         "adjdev": os.path.join(BUILD, "chns_adjoint.o"),
+        "filter": os.path.join(BUILD, "chns_filter.o"),
     }
     steps = [
         ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INC,
@@ -64,11 +65,13 @@ def build(verbose: bool = False) -> str:
          "-c", os.path.join(CSRC, "chns_adjoint.cpp"), "-o", objs["adjapi"]],
         [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
          "-c", os.path.join(CSRC, "chns_adjoint.hip"), "-o", objs["adjdev"]],
+        [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
+         "-c", os.path.join(CSRC, "chns_filter.hip"), "-o", objs["filter"]],
     ]
     with ThreadPoolExecutor(8) as ex:
         list(ex.map(_run, steps))
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["batch"],
-          objs["adjdev"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
+          objs["adjdev"], objs["filter"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
     # time-loop driver (plain C on the C ABI), found next to the library via rpath
     _run(["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-I" + INC, os.path.join(HERE, "host", "chns2d.c"),
           "-o", DRIVER, "-L" + os.path.dirname(OUT), "-lchns_amd", "-Wl,-rpath,$ORIGIN", "-lm"])

@@ -29,7 +29,7 @@ CHNS_OK, CHNS_E_ARG, CHNS_E_HIP, CHNS_E_STATE = 0, -1, -2, -3
 EXPORTS = (
     "chns_params_default", "chns_create", "chns_destroy", "chns_last_error", "chns_set_state",
     "chns_get_state", "chns_set_real_state", "chns_step", "chns_sync", "chns_truncate_state",
-    "chns_etd_tables", "chns_dealias_extent", "chns_timing_enable", "chns_timing_read",
+    "chns_etd_tables", "chns_dealias_extent", "chns_filter_table", "chns_timing_enable", "chns_timing_read",
     "chns_free_energy", "chns_kinetic_energy", "chns_perimeter", "chns_spectrum", "chns_real_field",
     "chns_diag_sizes", "chns_ensemble_create", "chns_ensemble_destroy", "chns_ensemble_last_error",
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
@@ -91,6 +91,7 @@ def load_library(path: str = LIB_PATH):
     lib.chns_truncate_state.argtypes = [V]
     lib.chns_etd_tables.argtypes = [P, D, D, D, D, D, D]
     lib.chns_dealias_extent.argtypes = [P, ctypes.POINTER(ctypes.c_int)]
+    lib.chns_filter_table.argtypes = [P, D]
     lib.chns_timing_enable.argtypes = [V, I]
     lib.chns_timing_read.argtypes = [V, D, ctypes.POINTER(ctypes.c_int)]
     lib.chns_ensemble_create.argtypes = [ctypes.POINTER(V), P, I, I]
@@ -426,6 +427,16 @@ def dealias_extent(p: CParams):
     if rc != CHNS_OK:
         raise ChnsError(f"chns_dealias_extent failed ({rc})")
     return a
+
+
+def filter_table(p: CParams):
+    """SMOOTH_FILTER factor exp(-36 pp^18) per mode (IDXC order), no device needed."""
+    lib = load_library()
+    out = np.zeros((p.Nx, p.Ny // 2 + 1))
+    rc = lib.chns_filter_table(ctypes.byref(p), _dp(out))
+    if rc != CHNS_OK:
+        raise ChnsError(f"chns_filter_table failed ({rc})")
+    return out
 
 
 def two_pi():

@@ -215,8 +215,9 @@ int check_ctx(chns_ctx* c) {
   if (!c) return CHNS_E_ARG;
   if (c->p.model != CHNS_MODEL_TERNARY) return chns_fail(c, CHNS_E_ARG, "the adjoint supports the ternary model");
   if (c->in_ensemble) return chns_fail(c, CHNS_E_ARG, "the adjoint runs on a stand-alone context");
-  if (c->p.gravity || c->p.vpm || c->p.sponge || c->p.f_amp != 0.0)
-    return chns_fail(c, CHNS_E_ARG, "the adjoint covers the plain ternary step (no gravity, vpm, sponge, forcing)");
+  if (c->p.gravity || c->p.vpm || c->p.sponge || c->p.smooth_filter || c->p.f_amp != 0.0)
+    return chns_fail(c, CHNS_E_ARG,
+                     "the adjoint covers the plain ternary step (no gravity, vpm, sponge, smooth_filter, forcing)");
   AHIP(c, hipSetDevice(c->dev));
   if (!c->adj) c->adj = new chns_adj_state();
   return ensure_work(c);
@@ -386,7 +387,7 @@ extern "C" int chns_adj_seed(chns_ctx* c, const double* wbar, const double* c1ba
       chns_to_cm(src[f], tmp, N, H);
       AHIP(c, hipMemcpyAsync(dst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
       AHIP(c, hipStreamSynchronize(c->stream));
-      ALAUNCH(c, chns::launch_truncate(dst, c->b.amask, N, c->stream));   // adjoint states live on the kept modes
+      ALAUNCH(c, chns::launch_truncate(dst, c->b.amask, nullptr, N, c->stream));   // adjoint states live on the kept modes
     } else {
       AHIP(c, hipMemsetAsync(dst, 0, a->S * sizeof(chns::cd), c->stream));
     }

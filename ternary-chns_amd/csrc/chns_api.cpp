@@ -241,6 +241,15 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
     b.chi = dchi;
     b.spg = dspg;
   }
+  if (p->smooth_filter) {
+    std::vector<double> ft(HH);
+    chns_host::build_filter_table(p, ft.data());
+    double* dft;
+    if ((rc = dalloc(c, &dft, HH))) return bail(rc);
+    if ((e = hipMemcpy(dft, ft.data(), HH * 8, hipMemcpyHostToDevice)) != hipSuccess)
+      return bail(hip_fail(c, e, "hipMemcpy(filter)"));
+    b.filt = dft;
+  }
   if (p->gravity && (rc = dalloc(c, &b.GR, S))) return bail(rc);
   if (p->sponge && (rc = dalloc(c, &b.Hw, RJ))) return bail(rc);
   b.tw = dtw;
@@ -304,8 +313,8 @@ extern "C" int chns_set_real_state(chns_ctx* c, const double* wz, const double* 
     if (bb) CHNS_HIP(c, hipMemcpyAsync(rb, bb, nb, hipMemcpyHostToDevice, c->stream));
     CHNS_LAUNCH(c, chns::launch_row_r2c(c->d.log2n, ra, bb ? rb : nullptr, c->b.G1, c->b.G2, c->b.tw, c->ldF,
                                         c->stream));
-    CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G1, oa, c->b.tw, c->b.amask, c->ldF, c->stream));
-    if (bb) CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G2, ob, c->b.tw, c->b.amask, c->ldF,
+    CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G1, oa, c->b.tw, c->b.amask, c->b.filt, c->ldF, c->stream));
+    if (bb) CHNS_LAUNCH(c, chns::launch_col_fwd(c->d.log2n, true, c->b.G2, ob, c->b.tw, c->b.amask, c->b.filt, c->ldF,
                                                 c->stream));
     CHNS_HIP(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -333,17 +342,30 @@ extern "C" int chns_step(chns_ctx* c, int nsteps) {
   hipStream_t s = c->stream;
   const chns::Bufs& B = c->b;
   const chns::KParams& K = c->kp;
+  // SMOOTH_FILTER: the passes of chns_filter.hip
+  const bool F = B.filt != nullptr;
   if (nsteps > 0 && c->nph > 0 && !c->hc_valid)
-    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLA, [&] { return chns::launch_colA(L, B, K, c->nph, s); }));
+    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLA, [&] {
+      return F ? chns::launch_filt_colA(L, B, K, c->nph, s) : chns::launch_colA(L, B, K, c->nph, s);
+    }));
   for (int n = 0; n < nsteps; ++n) {
     for (int st = 1; st <= 2; ++st) {
-      if (c->nph > 0) CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] { return chns::launch_rowB(L, M, B, K, s); }));
-      CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] { return chns::launch_colC(L, M, B, K, s); }));
-      CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] { return chns::launch_rowD(L, M, B, K, s); }));
+      if (c->nph > 0)
+        CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] {
+          return F ? chns::launch_filt_rowB(L, M, B, K, s) : chns::launch_rowB(L, M, B, K, s);
+        }));
+      CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] {
+        return F ? chns::launch_filt_colC(L, M, B, K, s) : chns::launch_colC(L, M, B, K, s);
+      }));
+      CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] {
+        return F ? chns::launch_filt_rowD(L, M, B, K, s) : chns::launch_rowD(L, M, B, K, s);
+      }));
       if (K.grav) CHNS_LAUNCH(c, chns::launch_gravity(B.C1, c->nph == 2 ? B.C2 : nullptr, B.GR, K.gr1, K.gr2, K.f,
                                                       c->d.N, s));
-      CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2,
-                           [&] { return chns::launch_colE(L, M, st, B, K, 1 + c->nph, s); }));
+      CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2, [&] {
+        return F ? chns::launch_filt_colE(L, M, st, B, K, 1 + c->nph, s)
+                 : chns::launch_colE(L, M, st, B, K, 1 + c->nph, s);
+      }));
     }
   }
   if (nsteps > 0) c->hc_valid = true;
@@ -361,8 +383,8 @@ extern "C" int chns_truncate_state(chns_ctx* c) {
   if (!c) return CHNS_E_ARG;
   if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
   CHNS_HIP(c, hipSetDevice(c->dev));
-  if (c->nph >= 1) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C1, c->b.amask, c->d.N, c->stream));
-  if (c->nph >= 2) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C2, c->b.amask, c->d.N, c->stream));
+  if (c->nph >= 1) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C1, c->b.amask, c->b.filt, c->d.N, c->stream));
+  if (c->nph >= 2) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C2, c->b.amask, c->b.filt, c->d.N, c->stream));
   return CHNS_OK;
 }
 

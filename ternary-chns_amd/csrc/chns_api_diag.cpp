@@ -52,10 +52,10 @@ const chns::cd* field(chns_ctx* c, int which) { return which == 0 ? c->b.W : (wh
 // Real-space pair (op_a on field a, op_b on field b; b < 0: none) into real buffers ra, rb.
 int real_pair(chns_ctx* c, int fa, int opa, int fb, int opb, bool trunc, int ra, int rb) {
   const int L = c->d.log2n;
-  DLAUNCH(c, chns::launch_col_inv_full(L, opa, trunc, field(c, fa), c->dT1, c->b.tw, c->b.amask, c->ldF,
+  DLAUNCH(c, chns::launch_col_inv_full(L, opa, trunc, field(c, fa), c->dT1, c->b.tw, c->b.amask, c->b.filt, c->ldF,
                                        c->d.factor_x, c->stream));
   if (fb >= 0)
-    DLAUNCH(c, chns::launch_col_inv_full(L, opb, trunc, field(c, fb), c->dT2, c->b.tw, c->b.amask, c->ldF,
+    DLAUNCH(c, chns::launch_col_inv_full(L, opb, trunc, field(c, fb), c->dT2, c->b.tw, c->b.amask, c->b.filt, c->ldF,
                                          c->d.factor_x, c->stream));
   DLAUNCH(c, chns::launch_row_c2r_full(L, c->dT1, fb >= 0 ? c->dT2 : nullptr, real_buf(c, ra),
                                        fb >= 0 ? real_buf(c, rb) : nullptr, c->b.tw, c->ldF,
@@ -156,6 +156,7 @@ extern "C" int chns_spectrum(chns_ctx* c, int truncate_state, double* ek, double
   A.C1 = c->b.C1;
   A.C2 = c->b.C2;
   A.amask = c->b.amask;
+  A.filt = c->b.filt;
   A.f = c->d.factor_x;
   A.dk = 2.0 * M_PI / c->p.lx;                          // spectrum.c:16
   A.kymax = (c->p.Ny / 2.0) * c->d.factor_y;            // main.c:58
@@ -197,7 +198,7 @@ int chns_real_of(chns_ctx* c, const chns::cd* S, double* out) {
   int rc = prepare(c);
   if (rc) return rc;
   const int L = c->d.log2n;
-  DLAUNCH(c, chns::launch_col_inv_full(L, chns::COP_ID, false, S, c->dT1, c->b.tw, c->b.amask, c->ldF,
+  DLAUNCH(c, chns::launch_col_inv_full(L, chns::COP_ID, false, S, c->dT1, c->b.tw, c->b.amask, nullptr, c->ldF,
                                        c->d.factor_x, c->stream));
   DLAUNCH(c, chns::launch_row_c2r_full(L, c->dT1, nullptr, real_buf(c, 0), nullptr, c->b.tw, c->ldF,
                                        1.0 / ((double)c->p.Nx * c->p.Ny), c->stream));

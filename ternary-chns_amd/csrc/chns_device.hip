@@ -17,10 +17,11 @@ int row_r2c(const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int 
   return (int)hipGetLastError();
 }
 template <int L>
-int col_fwd(bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF, hipStream_t s) {
+int col_fwd(bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, const double* ft, int ldF,
+            hipStream_t s) {
   const dim3 g(ceil_div<L, 16>((1 << L) / 2 + 1));
-  if (trunc) k_col_fwd<L, true><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ldF);
-  else k_col_fwd<L, false><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ldF);
+  if (trunc) k_col_fwd<L, true><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ft, ldF);
+  else k_col_fwd<L, false><<<g, ColCfg<L, 16>::NT, 0, s>>>(in, out, tw, amask, ft, ldF);
   return (int)hipGetLastError();
 }
 }  // namespace
@@ -44,13 +45,13 @@ int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, 
                    hipStream_t s) {
   CHNS_DISPATCH(log2n, (row_r2c<L>(a, b, oa, ob, tw, ldF, s)));
 }
-int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF,
-                   hipStream_t s) {
-  CHNS_DISPATCH(log2n, (col_fwd<L>(trunc, in, out, tw, amask, ldF, s)));
+int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, const double* ft,
+                   int ldF, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (col_fwd<L>(trunc, in, out, tw, amask, ft, ldF, s)));
 }
-int launch_truncate(cd* S, const int* amask, int N, hipStream_t s) {
+int launch_truncate(cd* S, const int* amask, const double* ft, int N, hipStream_t s) {
   const size_t n = (size_t)N * (N / 2 + 1);
-  k_truncate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(S, amask, N, N / 2 + 1);
+  k_truncate<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(S, amask, ft, N, N / 2 + 1);
   return (int)hipGetLastError();
 }
 int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, double f, int N, hipStream_t s) {
@@ -62,17 +63,18 @@ int col_tile(int log2n) { CHNS_DISPATCH(log2n, (ColCfg<L>::C)); }
 
 namespace {
 template <int L, int OP, bool TR>
-int col_inv_full_t(const cd* S, cd* out, const cd* tw, const int* amask, int ldF, double f, hipStream_t s) {
+int col_inv_full_t(const cd* S, cd* out, const cd* tw, const int* amask, const double* ft, int ldF, double f,
+                   hipStream_t s) {
   const dim3 g(round16(ceil_div<L>((1 << L) / 2 + 1)));
-  k_col_inv_full<L, OP, TR><<<g, ColCfg<L>::NT, 0, s>>>(S, out, tw, amask, ldF, f);
+  k_col_inv_full<L, OP, TR><<<g, ColCfg<L>::NT, 0, s>>>(S, out, tw, amask, ft, ldF, f);
   return (int)hipGetLastError();
 }
 template <int L>
-int col_inv_full(int op, bool tr, const cd* S, cd* out, const cd* tw, const int* amask, int ldF, double f,
-                 hipStream_t s) {
-#define CHNS_OPC(O)                                                          \
-  if (op == O) return tr ? col_inv_full_t<L, O, true>(S, out, tw, amask, ldF, f, s) \
-                         : col_inv_full_t<L, O, false>(S, out, tw, amask, ldF, f, s);
+int col_inv_full(int op, bool tr, const cd* S, cd* out, const cd* tw, const int* amask, const double* ft, int ldF,
+                 double f, hipStream_t s) {
+#define CHNS_OPC(O)                                                                \
+  if (op == O) return tr ? col_inv_full_t<L, O, true>(S, out, tw, amask, ft, ldF, f, s) \
+                         : col_inv_full_t<L, O, false>(S, out, tw, amask, ft, ldF, f, s);
   CHNS_OPC(COP_ID) CHNS_OPC(COP_IKX) CHNS_OPC(COP_IKY) CHNS_OPC(COP_U) CHNS_OPC(COP_V)
 #undef CHNS_OPC
   return (int)hipErrorInvalidValue;
@@ -86,8 +88,8 @@ int row_c2r_full(const cd* X, const cd* Y, double* oa, double* ob, const cd* tw,
 }  // namespace
 
 int launch_col_inv_full(int log2n, int op, bool trunc, const cd* S, cd* out, const cd* tw, const int* amask,
-                        int ldF, double f, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (col_inv_full<L>(op, trunc, S, out, tw, amask, ldF, f, s)));
+                        const double* ft, int ldF, double f, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (col_inv_full<L>(op, trunc, S, out, tw, amask, ft, ldF, f, s)));
 }
 int launch_row_c2r_full(int log2n, const cd* X, const cd* Y, double* oa, double* ob, const cd* tw, int ldF,
                         double scale, hipStream_t s) {

@@ -17,7 +17,8 @@ namespace chns {
 template <int LOG2N, int OP, bool TRUNC>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_inv_full(const cd* __restrict__ S, cd* out,
                                                                   const cd* __restrict__ tw,
-                                                                  const int* __restrict__ amask, int ldF,
+                                                                  const int* __restrict__ amask,
+                                                                  const double* __restrict__ ft, int ldF,
                                                                   double f) {
   using CF = ColCfg<LOG2N>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
@@ -43,6 +44,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_col_inv_full(const cd* __
       z = ks > 1e-12 ? mk(z.x / ks, z.y / ks) : mk(0.0, 0.0);
       z = OP == COP_U ? ctimes_ik(z, ky) : ctimes_ik(z, -kx);
     }
+    if (TRUNC && ft && ok) z = filt2(z, filt_at<LOG2N>(ft, j, i));   // SMOOTH_FILTER truncation
     v[m] = z;
   }
   fft<LOG2N, CF::P, true, CF::C>(v, lds, tw, t);
@@ -216,13 +218,18 @@ __global__ void k_spec_rows(SpecArgs A) {
       cur = ik;
     }
     const bool kept = ai <= A.amask[j];   // truncation of u and the phase fields (spectrum.c:22-27)
+    const double fe = A.filt ? A.filt[(size_t)j * H + ai] : 1.0;   // ... or the smooth filter
     const size_t idx = (size_t)j * N + i;
     double eu = 0.0;
     if (kept) {
       // get_velocity (velocity.c:20-39)
       cd w = A.W[idx];
       w = ks > 1e-12 ? mk(w.x / ks, w.y / ks) : mk(0.0, 0.0);
-      const cd ux = ctimes_ik(w, ky), uy = ctimes_ik(w, -kx);
+      cd ux = ctimes_ik(w, ky), uy = ctimes_ik(w, -kx);
+      if (A.filt) {
+        ux = filt2(ux, fe);
+        uy = filt2(uy, fe);
+      }
       eu = (ux.x * ux.x + ux.y * ux.y + uy.x * uy.x + uy.y * uy.y) * A.scale2;
     }
     const double ens = eu * ks;
@@ -233,10 +240,10 @@ __global__ void k_spec_rows(SpecArgs A) {
     ex += wh * eu;
     if (iky < A.nxy) A.party[(size_t)iky * N + i] = wh * eu;
     if (A.nph >= 1 && kept) {
-      const cd c1 = A.C1[idx];
+      const cd c1 = A.filt ? filt2(A.C1[idx], fe) : A.C1[idx];
       acc[2] += w2 * ((c1.x * c1.x + c1.y * c1.y) * A.scale2);
       if (A.nph >= 2) {
-        const cd c2 = A.C2[idx];
+        const cd c2 = A.filt ? filt2(A.C2[idx], fe) : A.C2[idx];
         acc[3] += w2 * ((c2.x * c2.x + c2.y * c2.y) * A.scale2);
         acc[4] += w2 * fabs((c1.x * c2.x + c1.y * c2.y) * A.scale2);
       }

@@ -201,9 +201,9 @@ extern "C" int chns_ensemble_create(chns_ensemble** out, const chns_params* p, i
     return bail(ehip(e, r, "hipEventCreate"));
   std::map<std::pair<int, int>, int> gid;   // (log2n, model) -> group
   for (int b = 0; b < n; ++b) {
-    if (p[b].gravity || p[b].vpm || p[b].sponge)
+    if (p[b].gravity || p[b].vpm || p[b].sponge || p[b].smooth_filter)
       return bail(efail(e, CHNS_E_ARG, "instance " + std::to_string(b) +
-                                           ": the optional physics (gravity, vpm, sponge) run on single contexts"));
+                                           ": the optional physics (gravity, vpm, sponge, smooth_filter) run on single contexts"));
     chns_ctx* c = nullptr;
     const int rc = chns_create_on(&c, &p[b], device, e->s0);
     if (rc != CHNS_OK) return bail(efail(e, rc, "instance " + std::to_string(b) + ": invalid parameters"));

@@ -21,5 +21,8 @@ void build_twiddles(int N, double* tw);
 void dealias_extent(const chns_params* p, int* amax);
 // wall profile chi[Nx] and sponge[Ny] (times 1/eta) of init.c:184-211
 void build_wall_tables(const chns_params* p, double* chi, double* sponge);
+// SMOOTH_FILTER factor exp(-36 pp^18) of truncation() (functions.c:54,70-73)
+// at (j, |i|): ft[j*H + |i|], H = Ny/2+1
+void build_filter_table(const chns_params* p, double* ft);
 
 }  // namespace chns_host

@@ -155,6 +155,32 @@ __device__ __forceinline__ void col_load_masked(const cd* __restrict__ S, int j,
   }
 }
 
+// SMOOTH_FILTER form of truncation(): every mode times e twice, e =
+// exp(-36 pp^18) from the host table at (j, |i|) (functions.c:70-73)
+__device__ __forceinline__ cd filt2(cd z, double e) { return cscale(cscale(z, e), e); }
+template <int LOG2N>
+__device__ __forceinline__ double filt_at(const double* __restrict__ ft, int j, int i) {
+  constexpr int N = 1 << LOG2N;
+  return ft[(size_t)j * (N / 2 + 1) + absidx<LOG2N>(i)];
+}
+// truncation() of a state column on load: the sharp cut (a = largest kept
+// |i|), or with FILT the smooth filter (a < 0 still marks a column outside)
+template <int LOG2N, int P, bool FILT>
+__device__ __forceinline__ void col_load_trunc(const cd* __restrict__ S, const double* __restrict__ ft, int j, int a,
+                                               int t, cd* v) {
+  if constexpr (FILT) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int T = N / P;
+#pragma unroll
+    for (int m = 0; m < P; ++m) {
+      const int i = t + m * T;
+      v[m] = a >= 0 ? filt2(S[(size_t)j * N + i], filt_at<LOG2N>(ft, j, i)) : mk(0.0, 0.0);
+    }
+  } else {
+    col_load_masked<LOG2N, P>(S, j, a, t, v);
+  }
+}
+
 // ---------------------------------------------------------------- row I/O
 // Assemble Z = X + iY (length N, Hermitian halves) from two dealiased
 // half-spectra rows so one complex c2r yields two real rows.  Only real parts
@@ -317,7 +343,7 @@ __device__ __forceinline__ void rfft_split(cd* v, cd* lds, const Tw& tws, int t,
 
 // ============================================================ pass A
 // grid: (ceil(Jt/C), nphase)
-template <int LOG2N, class Args>
+template <int LOG2N, class Args, bool FILT = false>
 __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
@@ -331,14 +357,14 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
   const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
   cd v[CF::P];
-  col_load_masked<LOG2N, CF::P>(S, j, ok ? B.amask[j] : -1, t, v);
-  fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+  col_load_trunc<LOG2N, CF::P, FILT>(S, B.filt, j, ok ? B.amask[j] : -1, t, v);
+  fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
   col_store_rm<LOG2N, CF::P, TJ>(H, P.ldJ, j, ok, t, v);
 }
 
 // ============================================================ pass B
 // grid: N / R
-template <int LOG2N, int MODEL, class Args>
+template <int LOG2N, int MODEL, class Args, bool FILT = false>
 __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
@@ -354,7 +380,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
   const cd* h1 = B.Hc1 + tix<TJ>(x, 0, P.ldJ);
   const cd* h2 = (MODEL == MODEL_TERNARY) ? B.Hc2 + tix<TJ>(x, 0, P.ldJ) : nullptr;
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+  rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     const double c1 = v[m].x * P.scale;
@@ -387,7 +413,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
 
 // ============================================================ pass C
 // grid: 1-D over (tile, group) items (col_item): ternary 5 groups, binary 4, NS 2.
-template <int LOG2N, int MODEL, class Args>
+template <int LOG2N, int MODEL, class Args, bool FILT = false>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
@@ -420,15 +446,16 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
       if (absidx<LOG2N>(i) <= a) {
         if (MODEL == MODEL_TERNARY) v[m] = cadd(v[m], cscale(s[m], mc * ks));
         else v[m] = cadd(cscale(s[m], P.mcb * ks), cscale(v[m], P.s24));
+        if constexpr (FILT) v[m] = filt2(v[m], filt_at<LOG2N>(B.filt, j, i));
       } else {
         v[m] = mk(0.0, 0.0);
       }
     }
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
-    col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
+    col_load_trunc<LOG2N, CF::P, FILT>(B.W, B.filt, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
@@ -436,42 +463,42 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
       s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
       v[m] = ctimes_ik(s[m], ky);
     }
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
-    col_load_masked<LOG2N, CF::P>(B.W, j, a, t, s);
+    col_load_trunc<LOG2N, CF::P, FILT>(B.W, B.filt, j, a, t, s);
     if (P.sponge) {
       // T(omega) itself for the sponge term (nonlin.c:50-54)
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = s[m];
-      fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+      fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
       col_store_rm<LOG2N, CF::P, TJ>(B.Hw, P.ldJ, j, ok, t, v);
     }
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], ky);
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
-    col_load_masked<LOG2N, CF::P>(B.C1, j, a, t, s);
+    col_load_trunc<LOG2N, CF::P, FILT>(B.C1, B.filt, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hdx1, P.ldJ, j, ok, t, v);
     if constexpr (MODEL == MODEL_TERNARY) {
-      col_load_masked<LOG2N, CF::P>(B.C2, j, a, t, s);
+      col_load_trunc<LOG2N, CF::P, FILT>(B.C2, B.filt, j, a, t, s);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-      fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+      fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
       col_store_rm<LOG2N, CF::P, TJ>(B.Hdx2, P.ldJ, j, ok, t, v);
     }
   }
@@ -479,7 +506,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
 
 // ============================================================ pass D
 // grid: N / R
-template <int LOG2N, int MODEL, class Args>
+template <int LOG2N, int MODEL, class Args, bool FILT = false>
 __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
@@ -494,16 +521,16 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
   const double sc = P.scale;
   cd v[8];
   RowTw<LOG2N, RC::R> tws;
-  tws.template init<RC::R == 1>(B.tw, t);   // the stager follows at one row per workgroup
+  tws.template init<RC::R == 1 && !FILT>(B.tw, t);   // the stager follows at one row per workgroup
   if constexpr (MODEL == MODEL_NS) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
     row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
-    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+    rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
 #pragma unroll
     for (int m = 0; m < 8; ++m) { u[m] = v[m].x * sc; w[m] = v[m].y * sc; }
     row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
-    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+    rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
     if (P.vpm) {
       // nlz, then -u chi/eta, -v chi/eta whose curl is subtracted (nonlin.c:62-95);
       // + omega sponge(y)/eta with the sponge (nonlin.c:50-54,68-70,97-99)
@@ -512,7 +539,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
       if (P.sponge) {
         row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
-        rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+        rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
 #pragma unroll
         for (int m = 0; m < 8; ++m) nl[m] += (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)];
       }
@@ -545,7 +572,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     // One row per workgroup (N = 2048): the field pairs are staged in LDS,
     // alternately in two buffers, so pair k+2 streams in while pairs k and
     // k+1 are transformed.
-    constexpr bool STG = RC::R == 1;
+    constexpr bool STG = RC::R == 1 && !FILT;   // the stager holds dealiased rows
     __shared__ cd stgA[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
     __shared__ cd stgB[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
     const RowStager<LOG2N> sa{stgA, P.Jt}, sb{stgB, P.Jt};
@@ -562,7 +589,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+      rfft<LOG2N, true, RC::R, !FILT, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double m1 = v[m].x * sc, m2 = v[m].y * sc;
@@ -582,7 +609,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+      rfft<LOG2N, true, RC::R, !FILT, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         u[m] = v[m].x * sc;
@@ -598,7 +625,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
-      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+      rfft<LOG2N, true, RC::R, !FILT, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -613,7 +640,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
-        rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+        rfft<LOG2N, true, RC::R, !FILT, !STG>(v, lds, tws, t);   // after a stager barrier
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const double dx = v[m].x * sc, dy = v[m].y * sc;
@@ -655,14 +682,14 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
     }
-    rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+    rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
     if (P.sponge) {
       // + omega sponge(y)/eta in nlz (nonlin.c:50-54,68-70,97-99)
       double nl[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) nl[m] = u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc);
       row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
-      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+      rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = mk(-(nl[m] + (v[m].x * sc) * B.spg[rpos<LOG2N, RC::R>(t, m)]), fx[m]);
     } else {
@@ -687,7 +714,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
 
 // ============================================================ pass E
 // grid: 1-D over (tile, field) items (col_item): omega, then phase field(s).
-template <int LOG2N, int MODEL, int STAGE, class Args>
+template <int LOG2N, int MODEL, int STAGE, class Args, bool FILT = false>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
@@ -769,8 +796,12 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
     const bool okA = j < P.Jt;
     const int a = okA ? B.amask[j] : -1;
 #pragma unroll
-    for (int m = 0; m < CF::P; ++m) v[m] = (okA && absidx<LOG2N>(t + m * T) <= a) ? s[m] : mk(0.0, 0.0);
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, B.tw, t);
+    for (int m = 0; m < CF::P; ++m) {
+      const int i = t + m * T;
+      if constexpr (FILT) v[m] = okA ? filt2(s[m], filt_at<LOG2N>(B.filt, j, i)) : mk(0.0, 0.0);
+      else v[m] = (okA && absidx<LOG2N>(i) <= a) ? s[m] : mk(0.0, 0.0);
+    }
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }
@@ -808,7 +839,8 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
 template <int LOG2N, bool TRUNC>
 __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* __restrict__ in, cd* out,
                                                               const cd* __restrict__ tw,
-                                                              const int* __restrict__ amask, int ldF) {
+                                                              const int* __restrict__ amask,
+                                                              const double* __restrict__ ft, int ldF) {
   using CF = ColCfg<LOG2N, 16>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
@@ -824,7 +856,8 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_col_fwd(const cd* _
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
       const int i = t + m * T;
-      out[(size_t)j * N + i] = (absidx<LOG2N>(i) <= a) ? v[m] : mk(0.0, 0.0);
+      const cd z = TRUNC && ft ? filt2(v[m], filt_at<LOG2N>(ft, j, i)) : v[m];   // SMOOTH_FILTER
+      out[(size_t)j * N + i] = (absidx<LOG2N>(i) <= a) ? z : mk(0.0, 0.0);
     }
   }
 }
@@ -845,14 +878,16 @@ static __global__ void k_gravity(const cd* __restrict__ C1, const cd* __restrict
   GR[e] = ctimes_ik(r, kx);
 }
 
-// In-place truncation of a column-major spectral field (spectrum.c:24-26).
+// In-place truncation of a column-major spectral field (spectrum.c:24-26):
+// the sharp cut, or with ft the smooth filter (functions.c:70-73).
 // grid: ceil(Ny2*N / 256)
-static __global__ void k_truncate(cd* S, const int* __restrict__ amask, int N, int H) {
+static __global__ void k_truncate(cd* S, const int* __restrict__ amask, const double* __restrict__ ft, int N, int H) {
   const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (size_t)N * H) return;
   const int j = (int)(e / N), i = (int)(e % N);
   const int ai = i <= N / 2 ? i : N - i;
-  if (ai > amask[j]) S[e] = mk(0.0, 0.0);
+  if (ft) S[e] = filt2(S[e], ft[(size_t)j * H + ai]);
+  else if (ai > amask[j]) S[e] = mk(0.0, 0.0);
 }
 
 }  // namespace chns

@@ -15,11 +15,20 @@ int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream
 int launch_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nfields, hipStream_t s);
 int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int ldF,
                    hipStream_t s);
-int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, int ldF,
-                   hipStream_t s);
-int launch_truncate(cd* S, const int* amask, int N, hipStream_t s);
+// trunc: truncation() on the way, the sharp cut (amask) or, with ft, the smooth filter
+int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, const int* amask, const double* ft,
+                   int ldF, hipStream_t s);
+int launch_truncate(cd* S, const int* amask, const double* ft, int N, hipStream_t s);
 int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, double f, int N, hipStream_t s);
 int col_tile(int log2n);   // columns per column-kernel workgroup
+
+// the passes with the SMOOTH_FILTER form of truncation() (chns_filter.hip;
+// single instances, every mode kept, Bufs::filt set)
+int launch_filt_colA(int log2n, const Bufs& B, const KParams& P, int nphase, hipStream_t s);
+int launch_filt_rowB(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_filt_colC(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_filt_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+int launch_filt_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nfields, hipStream_t s);
 
 // batched passes (chns_batch.hip): nb instances of one size and model, device
 // arrays B[nb], P[nb]
@@ -34,7 +43,7 @@ int launch_batch_colE(int log2n, int model, int stage, const Bufs* B, const KPar
 struct RedArgs;
 struct SpecArgs;
 int launch_col_inv_full(int log2n, int op, bool trunc, const cd* S, cd* out, const cd* tw, const int* amask,
-                        int ldF, double f, hipStream_t s);
+                        const double* ft, int ldF, double f, hipStream_t s);
 int launch_row_c2r_full(int log2n, const cd* X, const cd* Y, double* oa, double* ob, const cd* tw, int ldF,
                         double scale, hipStream_t s);
 int launch_reduce(int kind, const RedArgs& A, double* partial, double* out, hipStream_t s);

@@ -14,39 +14,39 @@ template <int L, int PP = 8> constexpr int ceil_div(int a) { return (a + ColCfg<
 inline int round16(int n) { return (n + 15) / 16 * 16; }
 inline int groupsC(int model) { return model == MODEL_TERNARY ? 5 : (model == MODEL_BINARY ? 4 : 2); }
 
-template <int L, class A> int colA(const A& a, int Jt, int nph, int nb, hipStream_t s) {
-  k_colA<L, A><<<dim3(ceil_div<L, 16>(Jt), nph, nb), ColCfg<L, 16>::NT, 0, s>>>(a);
+template <int L, bool FILT = false, class A> int colA(const A& a, int Jt, int nph, int nb, hipStream_t s) {
+  k_colA<L, A, FILT><<<dim3(ceil_div<L, 16>(Jt), nph, nb), ColCfg<L, 16>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
+template <int L, bool FILT = false, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
   const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
-  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else k_rowB<L, MODEL_BINARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else k_rowB<L, MODEL_BINARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
+template <int L, bool FILT = false, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
   const dim3 g(round16(ceil_div<L>(Jt) * groupsC(model)), 1, nb);
-  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else k_colC<L, MODEL_NS, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else k_colC<L, MODEL_NS, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
+template <int L, bool FILT = false, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
   const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
-  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else k_rowD<L, MODEL_NS, A><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
+  else k_rowD<L, MODEL_NS, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, int ST, class A> int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
+template <int L, int ST, bool FILT = false, class A> int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
   const dim3 g(round16(ceil_div<L>((1 << L) / 2 + 1) * nf), 1, nb);
-  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else k_colE<L, MODEL_NS, ST, A><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+  else k_colE<L, MODEL_NS, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, class A> int colE(int model, int stage, const A& a, int nf, int nb, hipStream_t s) {
-  return stage == 1 ? colE_st<L, 1>(model, a, nf, nb, s) : colE_st<L, 2>(model, a, nf, nb, s);
+template <int L, bool FILT = false, class A> int colE(int model, int stage, const A& a, int nf, int nb, hipStream_t s) {
+  return stage == 1 ? colE_st<L, 1, FILT>(model, a, nf, nb, s) : colE_st<L, 2, FILT>(model, a, nf, nb, s);
 }
 
 }  // namespace passes

@@ -10,6 +10,7 @@
 //  * Twiddles exp(-2 pi i n/N), evaluated in long double and rounded once.
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "chns.h"
 #include "chns_host.hpp"
@@ -69,7 +70,6 @@ const char* check_params(const chns_params* p) {
   if (l < 4 || l > 11) return "Nx must be a power of two in [16, 2048]";
   if (!(p->lx > 0.0)) return "lx must be positive";
   if (!(p->dt > 0.0)) return "dt must be positive";
-  if (p->smooth_filter) return "SMOOTH_FILTER is not supported by this build";
   if (p->f_amp != 0.0 && !(p->f_k > 0 && p->f_k < p->Ny / 2)) return "forcing f_k must lie in (0, Ny/2)";
   if (p->gravity && p->model == CHNS_MODEL_NS) return "gravity needs a phase field (BINARY or TERNARY)";
   if (p->sponge && !p->vpm) return "sponge needs vpm (its profile is set up with the walls, init.c:184-211)";
@@ -201,6 +201,11 @@ void build_twiddles(int N, double* tw) {
 void dealias_extent(const chns_params* p, int* amax) {
   Derived d;
   derive(p, &d);
+  if (p->smooth_filter) {
+    // the smooth filter zeroes no mode (functions.c:70-73)
+    for (int j = 0; j < d.H; ++j) amax[j] = p->Nx / 2;
+    return;
+  }
   const double dax = (p->Nx / 4.0) * d.factor_x;      // functions.c:56
   const double dealias = dax * dax;                    // functions.c:59
   for (int j = 0; j < d.H; ++j) {
@@ -212,6 +217,23 @@ void dealias_extent(const chns_params* p, int* amax) {
       if (!(ksqr > dealias)) a = i;                    // functions.c:75-78
     }
     amax[j] = a;
+  }
+}
+
+void build_filter_table(const chns_params* p, double* ft) {
+  Derived d;
+  derive(p, &d);
+  const double kxmax = (p->Nx / 2.0) * d.factor_x;             // main.c:57
+  const double kymax = (p->Ny / 2.0) * d.factor_y;             // main.c:58
+  const double ksqrmax = (double)(kxmax * kxmax + kymax * kymax);   // functions.c:54
+  for (int j = 0; j < d.H; ++j) {
+    const double ky = ky_of(d, j);
+    for (int i = 0; i <= p->Nx / 2; ++i) {
+      const double kx = kx_of(p, d, i);
+      const double ksqr = kx * kx + ky * ky;                   // functions.c:69
+      const double pp = ksqr / ksqrmax;                        // functions.c:71
+      ft[(size_t)j * d.H + i] = exp(-36.0 * pow(pp, 18));      // functions.c:72-73
+    }
   }
 }
 
@@ -235,6 +257,17 @@ extern "C" int chns_etd_tables(const chns_params* p, double* eta, double* z1w, d
       if (p->model != CHNS_MODEL_NS) { beta[o] = bb; z1c[o] = c1; z2c[o] = c2; }
     }
   }
+  return CHNS_OK;
+}
+
+extern "C" int chns_filter_table(const chns_params* p, double* ft) {
+  if (chns_host::check_params(p) || !ft) return CHNS_E_ARG;
+  chns_host::Derived d;
+  chns_host::derive(p, &d);
+  std::vector<double> t((size_t)d.H * d.H);
+  chns_host::build_filter_table(p, t.data());
+  for (int i = 0; i < p->Nx; ++i)
+    for (int j = 0; j < d.H; ++j) ft[(size_t)i * d.H + j] = t[(size_t)j * d.H + (i <= p->Nx / 2 ? i : p->Nx - i)];
   return CHNS_OK;
 }
 

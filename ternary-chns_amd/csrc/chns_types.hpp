@@ -47,6 +47,8 @@ struct Bufs {
   const double* tabw2;         // omega stage 2: z2
   const double* tabc2;         // phase stage 2: z2
   const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
+  const double* filt;          // SMOOTH_FILTER: [(N/2+1)*(N/2+1)] factor exp(-36 pp^18) at j*(N/2+1) + |i|,
+                               // applied twice wherever truncation() is (null: the sharp cut)
   const double* chi;           // [N] wall profile chi(x) (vpm)
   const double* spg;           // [N] sponge(y)/eta (sponge)
   cd* GR;                      // buoyancy torque, column-major (gravity)
@@ -70,6 +72,7 @@ struct RedArgs {
 struct SpecArgs {
   const cd *W, *C1, *C2;   // column-major state
   const int* amask;
+  const double* filt;      // SMOOTH_FILTER factor table (Bufs::filt) or null
   double f, dk, kymax, scale2;
   int N, nsh, nxy, nph;
   double* part;    // [5][nsh][N]

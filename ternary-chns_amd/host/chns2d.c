@@ -19,6 +19,7 @@
  *   model=ns|binary|ternary Nx lx nu dt maxiter nfile sigma epsilon lambda
  *   sigma_12 sigma_13 sigma_23 Xlambda f_amp f_k
  *   gravity rho1 rho2 rho3 g vpm sponge   (macro.h GRAVITY / VPM / SPONGE switches)
+ *   smooth_filter                         (macro.h SMOOTH_FILTER)
  *   init=noise|read|droplet|zero  nranks (noise: srand(10+2*rank) per slab)
  *   read1=path read2=path (raw float64 [Nx][Ny], INIT_READ_DATA; defaults
  *   init_lens/phi1.in, init_lens/phi2.in, binary init_field/phi.in)
@@ -102,6 +103,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "g")) p.g = atof(v);
     else if (!strcmp(k, "vpm")) p.vpm = atoi(v);           /* macro.h VPM */
     else if (!strcmp(k, "sponge")) p.sponge = atoi(v);     /* macro.h SPONGE */
+    else if (!strcmp(k, "smooth_filter")) p.smooth_filter = atoi(v);   /* macro.h SMOOTH_FILTER */
     else if (!strcmp(k, "init")) init = v;
     else if (!strcmp(k, "nranks")) nranks = atoi(v);
     else if (!strcmp(k, "read1")) read1 = v;
