@@ -412,7 +412,9 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
 }
 
 // ============================================================ pass C
-// grid: 1-D over (tile, group) items (col_item): ternary 5 groups, binary 4, NS 2.
+// grid: 1-D over (tile, group) items (col_item): ternary 6 groups, binary 4, NS 2.
+// Groups are dispatched in order, the one-transform groups (d/dx c_i) last,
+// so they fill the final round of workgroups.
 template <int LOG2N, int MODEL, class Args, bool FILT = false>
 __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
   const Bufs& B = inst_bufs(args);
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
   constexpr int N = CF::N, T = CF::T;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
-  constexpr int NG = MODEL == MODEL_TERNARY ? 5 : (MODEL == MODEL_BINARY ? 4 : 2);
+  constexpr int NG = MODEL == MODEL_TERNARY ? 6 : (MODEL == MODEL_BINARY ? 4 : 2);
   int tile, g;
   if (!col_item((P.Jt + CF::C - 1) / CF::C, NG, &tile, &g)) return;
   const int j = tile * CF::C + c;
@@ -488,19 +490,12 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
     fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
-    // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178)
-    col_load_trunc<LOG2N, CF::P, FILT>(B.C1, B.filt, j, a, t, s);
+    // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178): g = 4 c1, 5 c2
+    col_load_trunc<LOG2N, CF::P, FILT>(g == 4 ? B.C1 : B.C2, B.filt, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
-    col_store_rm<LOG2N, CF::P, TJ>(B.Hdx1, P.ldJ, j, ok, t, v);
-    if constexpr (MODEL == MODEL_TERNARY) {
-      col_load_trunc<LOG2N, CF::P, FILT>(B.C2, B.filt, j, a, t, s);
-#pragma unroll
-      for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-      fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
-      col_store_rm<LOG2N, CF::P, TJ>(B.Hdx2, P.ldJ, j, ok, t, v);
-    }
+    col_store_rm<LOG2N, CF::P, TJ>(g == 4 ? B.Hdx1 : B.Hdx2, P.ldJ, j, ok, t, v);
   }
 }
 

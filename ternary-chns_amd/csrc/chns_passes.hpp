@@ -12,7 +12,7 @@ namespace passes {
 
 template <int L, int PP = 8> constexpr int ceil_div(int a) { return (a + ColCfg<L, PP>::C - 1) / ColCfg<L, PP>::C; }
 inline int round16(int n) { return (n + 15) / 16 * 16; }
-inline int groupsC(int model) { return model == MODEL_TERNARY ? 5 : (model == MODEL_BINARY ? 4 : 2); }
+inline int groupsC(int model) { return model == MODEL_TERNARY ? 6 : (model == MODEL_BINARY ? 4 : 2); }
 
 template <int L, bool FILT = false, class A> int colA(const A& a, int Jt, int nph, int nb, hipStream_t s) {
   k_colA<L, A, FILT><<<dim3(ceil_div<L, 16>(Jt), nph, nb), ColCfg<L, 16>::NT, 0, s>>>(a);
