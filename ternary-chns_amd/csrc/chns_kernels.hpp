@@ -403,7 +403,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
       v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
     }
   }
-    cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
+  cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
   cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
   rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
     o1[TF * n] = a;
@@ -820,7 +820,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
     const size_t o = (size_t)x * N + rpos<LOG2N, RC::R>(t, m);
     v[m] = mk(a[o], b ? b[o] : 0.0);
   }
-    cd* pa = oa + tix<TF>(x, 0, ldF);
+  cd* pa = oa + tix<TF>(x, 0, ldF);
   cd* pb = ob ? ob + tix<TF>(x, 0, ldF) : nullptr;
   rfft_split<LOG2N, RC::R>(v, lds, tw, t, [&](int n, cd A, cd Bv) {
     pa[TF * n] = A;
