@@ -45,7 +45,9 @@ WRITE_SCALE = 1.0
 def kernel_tag(pass_name: str, N: int) -> str:
     """Substring of the demangled kernel name of a pass (ternary model)."""
     L = int(round(math.log2(N)))
-    inst = "chns::Inst>"   # single-instance launches (ensembles use chns::InstBatch)
+    # single-instance launches of the plain step: the last template argument is
+    # the SMOOTH_FILTER switch (ensembles use chns::InstBatch)
+    inst = "chns::Inst, false>"
     return {"colA": f"k_colA<{L}, {inst}", "rowB": f"k_rowB<{L}, 2, {inst}", "colC": f"k_colC<{L}, 2, {inst}",
             "rowD": f"k_rowD<{L}, 2, {inst}", "colE1": f"k_colE<{L}, 2, 1, {inst}",
             "colE2": f"k_colE<{L}, 2, 2, {inst}"}[pass_name]
