@@ -67,3 +67,20 @@ def test_invalid_params_rejected():
     p = C.params("ternary", Nx=64, Ny=32)
     assert lib.chns_dealias_extent(C.ctypes.byref(p), np.zeros(17, np.int32).ctypes.data_as(
         C.ctypes.POINTER(C.ctypes.c_int))) == C.CHNS_E_ARG
+
+
+def test_bench_kernel_tags_name_built_kernels():
+    """Each pass tag bench.py looks for in rocprofv3 output names exactly one
+    kernel of the library (the plain single-instance step at 2048^2)."""
+    import shutil
+    import subprocess
+    import bench
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not found")
+    out = subprocess.run([nm, "-C", C.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    kernels = {ln.split("__device_stub__", 1)[1] for ln in out.splitlines() if "__device_stub__" in ln}
+    for p in ("rowB", "colC", "rowD", "colE1", "colE2"):
+        tag = bench.kernel_tag(p, 2048)
+        hits = [k for k in kernels if tag in k]
+        assert len(hits) == 1, (p, tag, hits)
