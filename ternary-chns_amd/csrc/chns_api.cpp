@@ -148,6 +148,8 @@ extern "C" const char* chns_last_error(const chns_ctx* ctx) {
 extern "C" void chns_destroy(chns_ctx* c) {
   if (!c || c->in_ensemble) return;   // an ensemble's instances go with the ensemble
   chns_adj_free(c);
+  if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
+  if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (void* q : c->allocs) (void)hipFree(q);
   if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -333,42 +335,93 @@ extern "C" int chns_set_real_state(chns_ctx* c, const double* wz, const double* 
   return CHNS_OK;
 }
 
-extern "C" int chns_step(chns_ctx* c, int nsteps) {
-  if (!c) return CHNS_E_ARG;
-  if (nsteps < 0) return fail(c, CHNS_E_ARG, "nsteps < 0");
-  if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
-  CHNS_HIP(c, hipSetDevice(c->dev));
+namespace {
+
+// The launches of one ETD2RK step (etd2rk.c:14-224): two stages of passes
+// B, C, D (and the buoyancy term) and E.  Each pass is bracketed by timing
+// events when timing is on.
+int enqueue_step(chns_ctx* c) {
   const int L = c->d.log2n, M = c->p.model;
   hipStream_t s = c->stream;
   const chns::Bufs& B = c->b;
   const chns::KParams& K = c->kp;
   // SMOOTH_FILTER: the passes of chns_filter.hip
   const bool F = B.filt != nullptr;
-  if (nsteps > 0 && c->nph > 0 && !c->hc_valid)
-    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLA, [&] {
-      return F ? chns::launch_filt_colA(L, B, K, c->nph, s) : chns::launch_colA(L, B, K, c->nph, s);
+  for (int st = 1; st <= 2; ++st) {
+    if (c->nph > 0)
+      CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] {
+        return F ? chns::launch_filt_rowB(L, M, B, K, s) : chns::launch_rowB(L, M, B, K, s);
+      }));
+    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] {
+      return F ? chns::launch_filt_colC(L, M, B, K, s) : chns::launch_colC(L, M, B, K, s);
     }));
-  for (int n = 0; n < nsteps; ++n) {
-    for (int st = 1; st <= 2; ++st) {
-      if (c->nph > 0)
-        CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] {
-          return F ? chns::launch_filt_rowB(L, M, B, K, s) : chns::launch_rowB(L, M, B, K, s);
-        }));
-      CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] {
-        return F ? chns::launch_filt_colC(L, M, B, K, s) : chns::launch_colC(L, M, B, K, s);
-      }));
-      CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] {
-        return F ? chns::launch_filt_rowD(L, M, B, K, s) : chns::launch_rowD(L, M, B, K, s);
-      }));
-      if (K.grav) CHNS_LAUNCH(c, chns::launch_gravity(B.C1, c->nph == 2 ? B.C2 : nullptr, B.GR, K.gr1, K.gr2, K.f,
-                                                      c->d.N, s));
-      CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2, [&] {
-        return F ? chns::launch_filt_colE(L, M, st, B, K, 1 + c->nph, s)
-                 : chns::launch_colE(L, M, st, B, K, 1 + c->nph, s);
-      }));
-    }
+    CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] {
+      return F ? chns::launch_filt_rowD(L, M, B, K, s) : chns::launch_rowD(L, M, B, K, s);
+    }));
+    if (K.grav) CHNS_LAUNCH(c, chns::launch_gravity(B.C1, c->nph == 2 ? B.C2 : nullptr, B.GR, K.gr1, K.gr2, K.f,
+                                                    c->d.N, s));
+    CHNS_LAUNCH(c, timed(c, st == 1 ? CHNS_PASS_COLE1 : CHNS_PASS_COLE2, [&] {
+      return F ? chns::launch_filt_colE(L, M, st, B, K, 1 + c->nph, s)
+               : chns::launch_colE(L, M, st, B, K, 1 + c->nph, s);
+    }));
   }
-  if (nsteps > 0) c->hc_valid = true;
+  return CHNS_OK;
+}
+
+// Capture one step into c->step_exec (kernel nodes only: no memset or copy).
+int capture_step(chns_ctx* c) {
+  CHNS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(c);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc != CHNS_OK || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != CHNS_OK ? rc : hip_fail(c, e, "hipStreamEndCapture");
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(c, ei, "hipGraphInstantiate");
+  }
+  c->step_graph = graph;
+  c->step_exec = exec;
+  return CHNS_OK;
+}
+
+}  // namespace
+
+// Steps are replayed from a graph of one captured step; with event timing on,
+// or when the caller is capturing the stream into a graph of its own, the
+// passes are launched directly instead.
+extern "C" int chns_step(chns_ctx* c, int nsteps) {
+  if (!c) return CHNS_E_ARG;
+  if (nsteps < 0) return fail(c, CHNS_E_ARG, "nsteps < 0");
+  if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
+  if (nsteps == 0) return CHNS_OK;
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  const chns::Bufs& B = c->b;
+  const bool F = B.filt != nullptr;
+  if (c->nph > 0 && !c->hc_valid)
+    CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLA, [&] {
+      return F ? chns::launch_filt_colA(c->d.log2n, B, c->kp, c->nph, c->stream)
+               : chns::launch_colA(c->d.log2n, B, c->kp, c->nph, c->stream);
+    }));
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  CHNS_HIP(c, hipStreamIsCapturing(c->stream, &cap));
+  if (c->timing || cap != hipStreamCaptureStatusNone) {
+    for (int n = 0; n < nsteps; ++n) {
+      const int rc = enqueue_step(c);
+      if (rc != CHNS_OK) return rc;
+    }
+  } else {
+    if (!c->step_exec) {
+      const int rc = capture_step(c);
+      if (rc != CHNS_OK) return rc;
+    }
+    for (int n = 0; n < nsteps; ++n) CHNS_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+  }
+  c->hc_valid = true;
   return CHNS_OK;
 }
 
