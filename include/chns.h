@@ -85,11 +85,12 @@ int chns_set_real_state(chns_ctx *ctx, const double *wz, const double *phi1, con
  * nonlin() (nonlin.c:13-381), surface_tension() (surface_tension.c:13-227),
  * get_velocity() (velocity.c:3-44), curl() (curl.c:16-31), truncation()
  * (functions.c:47-83) and the dfft/ifft pairs (fft.c:11-31) inside them.
- * Returns when the work is queued; chns_sync() waits.  The first call
- * captures one step into a HIP graph, replayed once per step from then on;
- * with event timing on (chns_timing_enable), or while the caller captures
- * the context's stream into a graph of its own, the kernels are launched
- * directly. */
+ * Returns when the work is queued; chns_sync() waits.  Steps run 8 at a
+ * time as replays of a HIP graph of 8 captured steps (captured on the first
+ * call with nsteps >= 8), the remainder as direct launches; with event
+ * timing on (chns_timing_enable), or while the caller captures the
+ * context's stream into a graph of its own, every kernel is launched
+ * directly.  Both paths compute bit-identical states. */
 int chns_step(chns_ctx *ctx, int nsteps);
 int chns_sync(chns_ctx *ctx);
 

@@ -368,10 +368,17 @@ int enqueue_step(chns_ctx* c) {
   return CHNS_OK;
 }
 
-// Capture one step into c->step_exec (kernel nodes only: no memset or copy).
+// Steps per replay of the step graph.  Measured on MI355X against direct
+// launches: a graph of one step is 2-5 us/step slower at 128^2-1024^2, a
+// graph of 8 steps 3 % faster at 128^2 and equal from 512^2 up.
+constexpr int kGraphSteps = 8;
+
+// Capture kGraphSteps steps into c->step_exec (kernel nodes only: no memset
+// or copy).
 int capture_step(chns_ctx* c) {
   CHNS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  const int rc = enqueue_step(c);
+  int rc = CHNS_OK;
+  for (int k = 0; k < kGraphSteps && rc == CHNS_OK; ++k) rc = enqueue_step(c);
   hipGraph_t graph = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
   if (rc != CHNS_OK || e != hipSuccess) {
@@ -391,9 +398,10 @@ int capture_step(chns_ctx* c) {
 
 }  // namespace
 
-// Steps are replayed from a graph of one captured step; with event timing on,
-// or when the caller is capturing the stream into a graph of its own, the
-// passes are launched directly instead.
+// Steps are replayed kGraphSteps at a time from a graph of captured steps,
+// the remainder launched directly; with event timing on, or when the caller
+// is capturing the stream into a graph of its own, every pass is launched
+// directly.
 extern "C" int chns_step(chns_ctx* c, int nsteps) {
   if (!c) return CHNS_E_ARG;
   if (nsteps < 0) return fail(c, CHNS_E_ARG, "nsteps < 0");
@@ -415,11 +423,16 @@ extern "C" int chns_step(chns_ctx* c, int nsteps) {
       if (rc != CHNS_OK) return rc;
     }
   } else {
-    if (!c->step_exec) {
+    int n = 0;
+    if (nsteps >= kGraphSteps && !c->step_exec) {
       const int rc = capture_step(c);
       if (rc != CHNS_OK) return rc;
     }
-    for (int n = 0; n < nsteps; ++n) CHNS_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    for (; n + kGraphSteps <= nsteps; n += kGraphSteps) CHNS_HIP(c, hipGraphLaunch(c->step_exec, c->stream));
+    for (; n < nsteps; ++n) {
+      const int rc = enqueue_step(c);
+      if (rc != CHNS_OK) return rc;
+    }
   }
   c->hc_valid = true;
   return CHNS_OK;

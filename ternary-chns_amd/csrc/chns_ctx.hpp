@@ -23,8 +23,8 @@ struct chns_ctx {
   std::vector<int> amask;
   bool have_state = false, hc_valid = false;
   std::string err;
-  // one captured step of chns_step, replayed per step (buffers and parameters
-  // are fixed when the context is created, so it stays valid)
+  // captured steps of chns_step, replayed together (buffers and parameters
+  // are fixed when the context is created, so the graph stays valid)
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
   // event timing (chns_timing_enable)

@@ -1,10 +1,12 @@
-"""chns_step replays a captured one-step HIP graph; with event timing on it
-launches the passes directly.  Both paths run the same kernels in the same
-order on the same buffers, so they must agree bit for bit after every call:
-replays are interleaved with eager work (state reads, uploads, the spectrum
-truncation, diagnostics) and synchronisations, each call checked against the
-direct launches.  Parity of the step itself with the oracle is covered by the
-other GPU tests, which step through the graph (timing is off by default).
+"""chns_step replays a HIP graph of 8 captured steps (the remainder of a
+call launched directly); with event timing on it launches every pass
+directly.  Both paths run the same kernels in the same order on the same
+buffers, so they must agree bit for bit after every call: replays are
+interleaved with eager work (state reads, uploads, the spectrum truncation,
+diagnostics) and synchronisations, each call checked against the direct
+launches.  Parity of the step itself with the oracle is covered by the other
+GPU tests, which replay the graph whenever they take 8 steps or more (timing
+is off by default).
 """
 import math
 
@@ -50,7 +52,7 @@ def test_graph_replays_match_direct_launches(model, N, over):
         d.timing(True)
         for ctx in (g, d):
             ctx.set_state(*s0.fields())
-        for k in (1, 3, 2):
+        for k in (8, 11, 3, 16):
             g.step(k)
             d.step(k)
             same(g, d)
@@ -59,16 +61,16 @@ def test_graph_replays_match_direct_launches(model, N, over):
         # a state upload between replays (pass A runs again, outside the graph)
         for ctx in (g, d):
             ctx.set_state(*s0.fields())
-        g.step(2)
-        d.step(2)
+        g.step(9)
+        d.step(9)
         same(g, d)
         if model != "ns":
             # the in-place truncation of spectrum() and a diagnostic between replays
             for ctx in (g, d):
                 ctx.truncate_state()
                 ctx.kinetic_energy()
-            g.step(2)
-            d.step(2)
+            g.step(8)
+            d.step(8)
             same(g, d)
 
 
@@ -80,7 +82,7 @@ def test_graph_matches_oracle_after_reupload():
     ref = O.run(op, s0, 20)
     with C.Context(cp) as ctx:
         ctx.set_state(*s0.fields())
-        ctx.step(5)
+        ctx.step(8)
         ctx.set_real_state(*reals)   # a different state: the truncated one
         ctx.set_state(*s0.fields())
         ctx.step(20)
