@@ -121,19 +121,21 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   const int Jt = P.Jt;
   const AdjRowLoader<LOG2N> ld{{stg_all, Jt}, tix<TJ>(x, 0, P.ldJ), Jt, P.f};
   cd v[8];
+  RowTw<LOG2N, RC::R> tws;
+  tws.template init<STG>(A.tw, t);   // before the stager's first copy
   double xi[8], e1[8], e2[8];
   ld.template first<false, true, false, false>(A.Hxi, A.Het1, A.Het2, nullptr, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { xi[m] = v[m].x * sc; e1[m] = v[m].y * sc; }
   ld.template next<false, false, true, false>(A.Het2, nullptr, A.Hu, A.Hv, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) e2[m] = v[m].x * sc;
   // (u, v): w bars and the advective grad-c bars, each pair to a row r2c right away
   double u[8], w[8];
   ld.template next<false, true, true, false>(A.Hu, A.Hv, A.Hwx, A.Hwy, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     u[m] = v[m].x * sc;
@@ -143,7 +145,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   {
     cd* o2 = A.O[2] + fo;
     cd* o3 = A.O[3] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o2[TF * n] = a;
         o3[TF * n] = ctimes_ik(b, -P.f * (double)n);
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   {
     cd* o10 = A.O[10] + fo;
     cd* o12 = A.O[12] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) { o10[TF * n] = a; o12[TF * n] = b; }
     });
   }
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   {
     cd* o11 = A.O[11] + fo;
     cd* o13 = A.O[13] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o11[TF * n] = ctimes_ik(a, -P.f * (double)n);
         o13[TF * n] = ctimes_ik(b, -P.f * (double)n);
@@ -174,21 +176,21 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   // ubar = -xi w_x - eta_1 c1x - eta_2 c2x, vbar = -xi w_y - eta_1 c1y - eta_2 c2y
   double ub[8], vb[8];
   ld.template next<false, true, true, false>(A.Hwx, A.Hwy, A.Hdx1, A.Hc1, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] = -xi[m] * (v[m].x * sc);
     vb[m] = -xi[m] * (v[m].y * sc);
   }
   ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] += -e1[m] * (v[m].x * sc);
     vb[m] += -e1[m] * (v[m].y * sc);
   }
   ld.template next<true, true, false, true>(A.Hdx2, A.Hc2, nullptr, nullptr, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) {
     ub[m] += -e2[m] * (v[m].x * sc);
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   {
     cd* o0 = A.O[0] + fo;
     cd* o1 = A.O[1] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         o0[TF * n] = ctimes_ik(a, -P.f * (double)n);   // -i ky R(ubar)
         o1[TF * n] = b;                                 // R(vbar)
@@ -230,21 +232,23 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   const int Jt = P.Jt;
   const AdjRowLoader<LOG2N> ld{{stg_all, Jt}, tix<TJ>(x, 0, P.ldJ), Jt, P.f};
   cd v[8];
+  RowTw<LOG2N, RC::R> tws;
+  tws.template init<STG>(A.tw, t);   // before the stager's first copy
   double xix[8], xiy[8], Ab1[8], Ab2[8];
   ld.template first<true, true, true, true>(A.Hxix, A.Hxi, A.Hdx1, A.Hc1, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { xix[m] = v[m].x * sc; xiy[m] = v[m].y * sc; }
   ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab1[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<true, true, true, false>(A.Hdx2, A.Hc2, A.Hmu1, A.Hmu2, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab2[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<false, true, false, true>(A.Hmu1, A.Hmu2, nullptr, nullptr, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
   double sq[4] = {0.0, 0.0, 0.0, 0.0};
   double A1[8], A2[8];
 #pragma unroll
@@ -261,7 +265,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o4 = A.O[4] + fo;
     cd* o7 = A.O[7] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) { o4[TF * n] = a; o7[TF * n] = b; }
     });
   }
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o5 = A.O[5] + fo;
     cd* o6 = A.O[6] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o5[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc1 * __dmul_rn(ky, ky)));
@@ -283,7 +287,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o8 = A.O[8] + fo;
     cd* o9 = A.O[9] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o8[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc2 * __dmul_rn(ky, ky)));
@@ -369,13 +373,15 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
   const size_t ro = tix<TJ>(x, 0, P.ldJ);
   const double sc = P.scale;
   cd v[8];
+  RowTw<LOG2N, RC::R> tws;
+  tws.init(A.tw, t);
   double c1[8], c2[8];
   row_load_pair<LOG2N, false>(A.Hc1 + ro, A.Hc2 + ro, P.Jt, P.f, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { c1[m] = v[m].x * sc; c2[m] = v[m].y * sc; }
   row_load_pair<LOG2N, false>(A.Hmb1 + ro, A.Hmb2 + ro, P.Jt, P.f, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
   // fac, G1, G2, G3, g1i, g2i, g3i
   double sq[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -418,7 +424,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
     cd* o1 = A.Ocb1 + tix<TF>(x, 0, P.ldJ);
   cd* o2 = A.Ocb2 + tix<TF>(x, 0, P.ldJ);
   const int Jt = P.Jt;
-  rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
+  rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
     if (n < Jt) { o1[TF * n] = a; o2[TF * n] = b; }
   });
   block_sum<7, RC::NT>(sq, (double*)lds_all, A.part + blockIdx.x, A.mb);
