@@ -232,23 +232,23 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   const int Jt = P.Jt;
   const AdjRowLoader<LOG2N> ld{{stg_all, Jt}, tix<TJ>(x, 0, P.ldJ), Jt, P.f};
   cd v[8];
-  RowTw<LOG2N, RC::R> tws;
-  tws.template init<STG>(A.tw, t);   // before the stager's first copy
+  // twiddles loaded per transform: held ones (as in adj_rowD1) spill 48
+  // VGPRs here, against 16, and measured slower
   double xix[8], xiy[8], Ab1[8], Ab2[8];
   ld.template first<true, true, true, true>(A.Hxix, A.Hxi, A.Hdx1, A.Hc1, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) { xix[m] = v[m].x * sc; xiy[m] = v[m].y * sc; }
   ld.template next<true, true, true, false>(A.Hdx1, A.Hc1, A.Hdx2, A.Hc2, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab1[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<true, true, true, false>(A.Hdx2, A.Hc2, A.Hmu1, A.Hmu2, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
 #pragma unroll
   for (int m = 0; m < 8; ++m) Ab2[m] = xiy[m] * (v[m].x * sc) - xix[m] * (v[m].y * sc);
   ld.template next<false, true, false, true>(A.Hmu1, A.Hmu2, nullptr, nullptr, t, v);
-  rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+  rfft<LOG2N, true, RC::R, true>(v, lds, A.tw, t);
   double sq[4] = {0.0, 0.0, 0.0, 0.0};
   double A1[8], A2[8];
 #pragma unroll
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o4 = A.O[4] + fo;
     cd* o7 = A.O[7] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) { o4[TF * n] = a; o7[TF * n] = b; }
     });
   }
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o5 = A.O[5] + fo;
     cd* o6 = A.O[6] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o5[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc1 * __dmul_rn(ky, ky)));
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   {
     cd* o8 = A.O[8] + fo;
     cd* o9 = A.O[9] + fo;
-    rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
+    rfft_split<LOG2N, RC::R>(v, lds, A.tw, t, [&](int n, cd a, cd b) {
       if (n < Jt) {
         const double ky = P.f * (double)n;
         o8[TF * n] = cadd(ctimes_ik(a, -ky), cscale(b, P.mc2 * __dmul_rn(ky, ky)));
