@@ -84,3 +84,13 @@ def test_bench_kernel_tags_name_built_kernels():
         tag = bench.kernel_tag(p, 2048)
         hits = [k for k in kernels if tag in k]
         assert len(hits) == 1, (p, tag, hits)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message, before any device call."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
