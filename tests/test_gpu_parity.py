@@ -247,3 +247,25 @@ def test_config_ternary_forced_1024():
         g = ctx.get_state()
     for a, b in zip(g, ref.fields()):
         assert rel(a, b) <= TOL_TRAJ
+
+
+@pytest.mark.parametrize("c0", [0.3, 0.85])
+def test_binary_linear_growth_rate(c0):
+    """One library step of c = c0 + d cos(3x + 2y), w = 0, grows the mode by
+    exp(s dt), s = -lambda k^2 (1.5 sigma eps k^2 + 24 (sigma/eps) P'(c0)),
+    P(c) = c(1-c)(1-2c) (linear stability of binary Cahn-Hilliard)."""
+    N, L = 32, 2 * math.pi
+    kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=1e-3, lam=1e-3, epsilon=1.5 * L / N, sigma=0.1)
+    op = O.Params(model="binary", **kw)
+    x = (np.arange(N) * op.dx)[:, None] * np.ones((1, N))
+    y = (np.arange(N) * op.dy)[None, :] * np.ones((N, 1))
+    s0 = O.State(np.zeros((N, N // 2 + 1), np.complex128), O.dfft(c0 + 1e-9 * np.cos(3 * x + 2 * y)))
+    with C.Context(C.params("binary", **kw)) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.step(1)
+        g = ctx.get_state()
+    dp = 1.0 - 6.0 * c0 + 6.0 * c0 * c0
+    rate = -1e-3 * 13 * (1.5 * 0.1 * op.epsilon * 13 + 24.0 * (0.1 / op.epsilon) * dp)
+    got = g[1][3, 2] / s0.cphi1[3, 2]
+    assert abs((got.real - 1.0) - math.expm1(rate * 1e-3)) < 1e-6 * abs(math.expm1(rate * 1e-3))
+    assert abs(got.imag) < 1e-14

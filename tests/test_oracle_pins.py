@@ -271,6 +271,34 @@ def test_forced_kolmogorov_steady_state():
     assert O.rel_l2(O.ifft(p, s.cwz), amp * np.sin(4 * y)) < 1e-12
 
 
+@pytest.mark.parametrize("c0", [0.3, 0.5, 0.85])
+def test_binary_linear_growth_rate(c0):
+    """Linear stability of binary Cahn-Hilliard at rest: a small mode
+    c = c0 + d cos(k.x) evolves as exp(s dt) with
+    s = -lambda k^2 (1.5 sigma eps k^2 + 24 (sigma/eps) P'(c0)), P(c) = c(1-c)(1-2c).
+    The quadratic term of P feeds only modes 0 and 2k and the cubic one is
+    O(d^2) relative, so one ETD2RK step of the oracle must give that factor to
+    the scheme's O((s dt)^3) error; a wrong weight, sign or power of k in the
+    interface or bulk term changes s at O(1)."""
+    p = O.Params(model="binary", Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, nu=5e-3, dt=1e-3, lam=1e-3,
+                 epsilon=1.5 * TWO_PI / 32, sigma=0.1)
+    x, y = grid(p)
+    kx, ky = 3, 2
+    d = 1e-9
+    s0 = O.State(np.zeros((32, 17), np.complex128), O.dfft(c0 + d * np.cos(kx * x + ky * y)))
+    s1 = O.etd2rk(p, s0)
+    k2 = kx * kx + ky * ky
+    dp = 1.0 - 6.0 * c0 + 6.0 * c0 * c0
+    rate = -p.lam * k2 * (1.5 * p.sigma * p.epsilon * k2 + 24.0 * (p.sigma / p.epsilon) * dp)
+    got = s1.cphi1[kx, ky] / s0.cphi1[kx, ky]
+    assert abs(got.imag) < 1e-15
+    # (got - 1) is about rate*dt ~ 1e-5..1e-4; compare it to 1e-6 of itself
+    assert abs((got.real - 1.0) - math.expm1(rate * p.dt)) < 1e-6 * abs(math.expm1(rate * p.dt))
+    # the mean stays; a single mode drives no flow (grad mu x grad c = 0), up to round-off
+    assert s1.cphi1[0, 0] == s0.cphi1[0, 0]
+    assert np.max(np.abs(s1.cwz)) < 1e-12 * abs(s0.cphi1[kx, ky])
+
+
 @pytest.mark.parametrize("model", ["binary", "ternary"])
 def test_mass_conservation(model):
     p = O.Params(model=model, Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI, nu=5e-3, dt=5e-4, lam=1e-3,
