@@ -867,7 +867,8 @@ PINS = {
     "surface_tension": "sympy real-space closed form for band-limited fields; 1-D fields give zero",
     "nonlin": "sympy closed form of u.grad w, u.grad c and lap(muF); Taylor-Green zero advection",
     "etd_tables": "closed form where well conditioned; bit-compared with the C tables (tests/test_abi.py)",
-    "etd2rk": "Stokes decay, forced Kolmogorov steady state, mass conservation",
+    "etd2rk": "Stokes decay, forced Kolmogorov steady state, mass conservation, "
+              "binary Cahn-Hilliard linear growth rate of one step",
     "free_energy": "binary closed form; d/dh F = <mu, dc> against the chemical potential; "
                    "ternary reduces to binary on two-phase states",
     "kinetic_energy": "single-mode closed form A^2/(4k^2), per-phase means",
