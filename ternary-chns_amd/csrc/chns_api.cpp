@@ -264,19 +264,35 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   return CHNS_OK;
 }
 
+namespace {
+
+// Device buffer for one field in the reference layout, allocated on first use.
+int stage_buffer(chns_ctx* c) {
+  if (c->dstage) return CHNS_OK;
+  return chns_dalloc_bytes(c, (void**)&c->dstage, (size_t)c->d.N * c->d.H * sizeof(chns::cd));
+}
+
+}  // namespace
+
+// The state crosses the boundary in the reference layout IDXC [Nx][Ny/2+1]
+// (global.h:13): each field is copied as it is into a device staging buffer
+// (or out of it) and transposed to or from the column-major [Ny/2+1][Nx]
+// layout on the device.
 extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi1, const double* cphi2) {
   if (!c) return CHNS_E_ARG;
   if (!cwz || (c->nph >= 1 && !cphi1) || (c->nph >= 2 && !cphi2)) return fail(c, CHNS_E_ARG, "null state array");
   CHNS_HIP(c, hipSetDevice(c->dev));
+  const int rc = stage_buffer(c);
+  if (rc != CHNS_OK) return rc;
   const int N = c->d.N, H = c->d.H;
-  std::vector<double> tmp;
+  const size_t bytes = (size_t)N * H * sizeof(chns::cd);
   const double* src[3] = {cwz, cphi1, cphi2};
   chns::cd* dst[3] = {c->b.W, c->b.C1, c->b.C2};
   for (int f = 0; f <= c->nph; ++f) {
-    chns_to_cm(src[f], tmp, N, H);
-    CHNS_HIP(c, hipMemcpyAsync(dst[f], tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
-    CHNS_HIP(c, hipStreamSynchronize(c->stream));
+    CHNS_HIP(c, hipMemcpyAsync(c->dstage, src[f], bytes, hipMemcpyHostToDevice, c->stream));
+    CHNS_LAUNCH(c, chns::launch_transpose(c->dstage, dst[f], N, H, c->stream));
   }
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
   c->hc_valid = false;
   return CHNS_OK;
@@ -286,16 +302,18 @@ extern "C" int chns_get_state(chns_ctx* c, double* cwz, double* cphi1, double* c
   if (!c) return CHNS_E_ARG;
   if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
   CHNS_HIP(c, hipSetDevice(c->dev));
+  const int rc = stage_buffer(c);
+  if (rc != CHNS_OK) return rc;
   const int N = c->d.N, H = c->d.H;
-  std::vector<double> tmp((size_t)2 * N * H);
+  const size_t bytes = (size_t)N * H * sizeof(chns::cd);
   double* dsts[3] = {cwz, cphi1, cphi2};
-  chns::cd* src[3] = {c->b.W, c->b.C1, c->b.C2};
+  const chns::cd* src[3] = {c->b.W, c->b.C1, c->b.C2};
   for (int f = 0; f <= c->nph; ++f) {
     if (!dsts[f]) continue;
-    CHNS_HIP(c, hipMemcpyAsync(tmp.data(), src[f], tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    CHNS_HIP(c, hipStreamSynchronize(c->stream));
-    chns_from_cm(tmp, dsts[f], N, H);
+    CHNS_LAUNCH(c, chns::launch_transpose(src[f], c->dstage, H, N, c->stream));
+    CHNS_HIP(c, hipMemcpyAsync(dsts[f], c->dstage, bytes, hipMemcpyDeviceToHost, c->stream));
   }
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
   return CHNS_OK;
 }
 

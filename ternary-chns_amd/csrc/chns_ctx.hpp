@@ -37,6 +37,7 @@ struct chns_ctx {
   double* dred = nullptr;                    // reduction partials + result
   double* dspec = nullptr;                   // spectrum row partials
   double* dspec_out = nullptr;               // spectrum sums
+  chns::cd* dstage = nullptr;                // one field in the IDXC layout (state transfer)
   struct chns_adj_state* adj = nullptr;      // discrete-adjoint storage (chns_adjoint.cpp)
 };
 

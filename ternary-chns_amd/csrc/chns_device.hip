@@ -59,6 +59,11 @@ int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, d
   k_gravity<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(C1, C2, GR, gr1, gr2, f, N);
   return (int)hipGetLastError();
 }
+int launch_transpose(const cd* in, cd* out, int rows, int cols, hipStream_t s) {
+  const dim3 g((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+  k_transpose<<<g, 256, 0, s>>>(in, out, rows, cols);
+  return (int)hipGetLastError();
+}
 int col_tile(int log2n) { CHNS_DISPATCH(log2n, (ColCfg<L>::C)); }
 
 namespace {

@@ -885,4 +885,28 @@ static __global__ void k_truncate(cd* S, const int* __restrict__ amask, const do
   else if (ai > amask[j]) S[e] = mk(0.0, 0.0);
 }
 
+// out[c][r] = in[r][c] for a row-major [rows][cols] complex array: the state
+// between the reference layout IDXC [Nx][Ny/2+1] and the column-major
+// [Ny/2+1][Nx] the passes use.  32 x 32 tiles through LDS (one padding
+// column, so the transposed reads hit distinct banks); both the loads and the
+// stores of a 32-lane row are 512 contiguous bytes.
+// grid: (ceil(cols/32), ceil(rows/32)), 256 threads
+static __global__ __launch_bounds__(256) void k_transpose(const cd* __restrict__ in, cd* __restrict__ out, int rows,
+                                                          int cols) {
+  __shared__ cd tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int r = r0 + ty + k, c = c0 + tx;
+    if (r < rows && c < cols) tile[ty + k][tx] = in[(size_t)r * cols + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 32; k += 8) {
+    const int c = c0 + ty + k, r = r0 + tx;
+    if (r < rows && c < cols) out[(size_t)c * rows + r] = tile[tx][ty + k];
+  }
+}
+
 }  // namespace chns

@@ -20,6 +20,8 @@ int launch_col_fwd(int log2n, bool trunc, const cd* in, cd* out, const cd* tw, c
                    int ldF, hipStream_t s);
 int launch_truncate(cd* S, const int* amask, const double* ft, int N, hipStream_t s);
 int launch_gravity(const cd* C1, const cd* C2, cd* GR, double gr1, double gr2, double f, int N, hipStream_t s);
+// out[c][r] = in[r][c], row-major [rows][cols] complex in, [cols][rows] out
+int launch_transpose(const cd* in, cd* out, int rows, int cols, hipStream_t s);
 int col_tile(int log2n);   // columns per column-kernel workgroup
 
 // the passes with the SMOOTH_FILTER form of truncation() (chns_filter.hip;
