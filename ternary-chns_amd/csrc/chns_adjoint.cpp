@@ -380,13 +380,13 @@ extern "C" int chns_adj_seed(chns_ctx* c, const double* wbar, const double* c1ba
   if (!a->forward_done) return chns_fail(c, CHNS_E_STATE, "chns_adj_forward first");
   const int N = c->d.N, H = c->d.H;
   const double* src[3] = {wbar, c1bar, c2bar};
-  std::vector<double> tmp;
+  if ((rc = chns_stage_buffer(c))) return rc;
   for (int f = 0; f < 3; ++f) {
     chns::cd* dst = a->ub + f * a->S;
     if (src[f]) {
-      chns_to_cm(src[f], tmp, N, H);
-      AHIP(c, hipMemcpyAsync(dst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c->stream));
-      AHIP(c, hipStreamSynchronize(c->stream));
+      // IDXC in, transposed to the column-major state on the device
+      AHIP(c, hipMemcpyAsync(c->dstage, src[f], a->S * sizeof(chns::cd), hipMemcpyHostToDevice, c->stream));
+      ALAUNCH(c, chns::launch_transpose(c->dstage, dst, N, H, c->stream));
       ALAUNCH(c, chns::launch_truncate(dst, c->b.amask, nullptr, N, c->stream));   // adjoint states live on the kept modes
     } else {
       AHIP(c, hipMemsetAsync(dst, 0, a->S * sizeof(chns::cd), c->stream));
@@ -453,14 +453,14 @@ extern "C" int chns_adj_get(chns_ctx* c, double* wbar, double* c1bar, double* c2
   chns_adj_state* a = c->adj;
   if (!a->backward_done) return chns_fail(c, CHNS_E_STATE, "no completed backward sweep");
   const int N = c->d.N, H = c->d.H;
-  std::vector<double> tmp((size_t)2 * a->S);
   double* dst[3] = {wbar, c1bar, c2bar};
+  if ((rc = chns_stage_buffer(c))) return rc;
   for (int f = 0; f < 3; ++f) {
     if (!dst[f]) continue;
-    AHIP(c, hipMemcpyAsync(tmp.data(), a->ub + f * a->S, tmp.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    AHIP(c, hipStreamSynchronize(c->stream));
-    chns_from_cm(tmp, dst[f], N, H);
+    ALAUNCH(c, chns::launch_transpose(a->ub + f * a->S, c->dstage, H, N, c->stream));
+    AHIP(c, hipMemcpyAsync(dst[f], c->dstage, a->S * sizeof(chns::cd), hipMemcpyDeviceToHost, c->stream));
   }
+  AHIP(c, hipStreamSynchronize(c->stream));
   if (partials) {
     AHIP(c, hipMemcpyAsync(partials, a->acc, chns::ADJ_NQ * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AHIP(c, hipStreamSynchronize(c->stream));

@@ -59,23 +59,6 @@ template <class T> int dalloc(chns_ctx* c, T** p, size_t count) {
 
 }  // namespace
 
-// host IDXC [N][H] complex <-> device column-major [H][N]
-void chns_to_cm(const double* src, std::vector<double>& dst, int N, int H) {
-  dst.resize((size_t)2 * N * H);
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j < H; ++j) {
-      dst[2 * ((size_t)j * N + i)] = src[2 * ((size_t)i * H + j)];
-      dst[2 * ((size_t)j * N + i) + 1] = src[2 * ((size_t)i * H + j) + 1];
-    }
-}
-void chns_from_cm(const std::vector<double>& src, double* dst, int N, int H) {
-  for (int i = 0; i < N; ++i)
-    for (int j = 0; j < H; ++j) {
-      dst[2 * ((size_t)i * H + j)] = src[2 * ((size_t)j * N + i)];
-      dst[2 * ((size_t)i * H + j) + 1] = src[2 * ((size_t)j * N + i) + 1];
-    }
-}
-
 namespace {
 
 void fill_kparams(chns_ctx* c) {
@@ -264,15 +247,11 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   return CHNS_OK;
 }
 
-namespace {
-
 // Device buffer for one field in the reference layout, allocated on first use.
-int stage_buffer(chns_ctx* c) {
+int chns_stage_buffer(chns_ctx* c) {
   if (c->dstage) return CHNS_OK;
   return chns_dalloc_bytes(c, (void**)&c->dstage, (size_t)c->d.N * c->d.H * sizeof(chns::cd));
 }
-
-}  // namespace
 
 // The state crosses the boundary in the reference layout IDXC [Nx][Ny/2+1]
 // (global.h:13): each field is copied as it is into a device staging buffer
@@ -282,7 +261,7 @@ extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi
   if (!c) return CHNS_E_ARG;
   if (!cwz || (c->nph >= 1 && !cphi1) || (c->nph >= 2 && !cphi2)) return fail(c, CHNS_E_ARG, "null state array");
   CHNS_HIP(c, hipSetDevice(c->dev));
-  const int rc = stage_buffer(c);
+  const int rc = chns_stage_buffer(c);
   if (rc != CHNS_OK) return rc;
   const int N = c->d.N, H = c->d.H;
   const size_t bytes = (size_t)N * H * sizeof(chns::cd);
@@ -302,7 +281,7 @@ extern "C" int chns_get_state(chns_ctx* c, double* cwz, double* cphi1, double* c
   if (!c) return CHNS_E_ARG;
   if (!c->have_state) return fail(c, CHNS_E_STATE, "no state set");
   CHNS_HIP(c, hipSetDevice(c->dev));
-  const int rc = stage_buffer(c);
+  const int rc = chns_stage_buffer(c);
   if (rc != CHNS_OK) return rc;
   const int N = c->d.N, H = c->d.H;
   const size_t bytes = (size_t)N * H * sizeof(chns::cd);

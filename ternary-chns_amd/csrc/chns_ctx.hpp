@@ -46,9 +46,9 @@ int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what);
 int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes);
 // chns_create on a caller-owned stream (nullptr: the context creates its own)
 int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t shared);
-// host IDXC [N][H] complex <-> device column-major [H][N] staging
-void chns_to_cm(const double* src, std::vector<double>& dst, int N, int H);
-void chns_from_cm(const std::vector<double>& src, double* dst, int N, int H);
+// c->dstage: one field in the reference layout IDXC [N][H], which
+// chns::launch_transpose turns into or out of the column-major [H][N] state
+int chns_stage_buffer(chns_ctx* c);
 // packed real [N][N] c2r (scaled 1/(Nx*Ny)) of a column-major spectral field S
 // of this context, copied to host memory out (synchronises)
 int chns_real_of(chns_ctx* c, const chns::cd* S, double* out);
