@@ -322,7 +322,7 @@ def main():
         print(json.dumps(ensemble_bench(a.steps if a.steps != 200 else 500, a.warmup)))
         return
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {a.gpus}); "
+        print(f"bench.py: measures one GPU only (got --gpus {a.gpus}); "
               "a single 2048^2 grid is not sharded across GPUs here", file=sys.stderr)
         sys.exit(2)
 

@@ -64,8 +64,8 @@ static int ilog2(int n) {
 const char* check_params(const chns_params* p) {
   if (!p) return "null params";
   if (p->model < 0 || p->model > 2) return "model must be 0 (NS), 1 (BINARY) or 2 (TERNARY)";
-  if (p->Nx != p->Ny) return "this build requires Nx == Ny";
-  if (p->lx != p->ly) return "this build requires lx == ly";
+  if (p->Nx != p->Ny) return "the library requires Nx == Ny";
+  if (p->lx != p->ly) return "the library requires lx == ly";
   const int l = ilog2(p->Nx);
   if (l < 4 || l > 11) return "Nx must be a power of two in [16, 2048]";
   if (!(p->lx > 0.0)) return "lx must be positive";

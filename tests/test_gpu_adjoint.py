@@ -2,7 +2,7 @@
 
 Checks, in order of independence from the HIP code:
 * the library gradient against central finite differences of the library's
-  own forward step (1e-6 relative, the north-star bar), including sigma;
+  own forward step (1e-6 relative), including sigma;
 * the library's reverse step and full reverse sweep against the oracle's
   adjoint (oracle/chns_adjoint.py, itself pinned by finite differences of the
   oracle forward) at 1e-10;

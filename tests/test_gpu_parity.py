@@ -1,6 +1,6 @@
 """GPU parity of the fused ETD2RK step against the oracle (through the C ABI).
 
-Bar (north star): relative L2 of omega, c1, c2 <= 1e-10 after 100 steps,
+Bar: relative L2 of omega, c1, c2 <= 1e-10 after 100 steps,
 mass (the k = 0 mode of each phase field) conserved to 1e-13.
 """
 import math
