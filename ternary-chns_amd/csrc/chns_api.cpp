@@ -130,6 +130,9 @@ extern "C" const char* chns_last_error(const chns_ctx* ctx) {
 
 extern "C" void chns_destroy(chns_ctx* c) {
   if (!c || c->in_ensemble) return;   // an ensemble's instances go with the ensemble
+  // queued work (graph replays included) finishes before anything it uses goes
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   chns_adj_free(c);
   if (c->step_exec) (void)hipGraphExecDestroy(c->step_exec);
   if (c->step_graph) (void)hipGraphDestroy(c->step_graph);
