@@ -143,16 +143,19 @@ def test_larger_grids_10_steps(model, N):
         assert rel(a - a0, b - a0) < 1e-11
 
 
-@pytest.mark.parametrize("model", ["binary", "ternary"])
+@pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
 def test_full_size_one_step(model):
-    """2048^2 (one row per row-kernel workgroup, LDS-staged inputs): one step vs the oracle."""
+    """2048^2 (one row per row-kernel workgroup; LDS-staged inputs for the phase
+    models, direct row loads for NS): one step vs the oracle."""
     N = 2048
     op, cp, _ = case(model, N, nu=2e-3, lam=5e-4, dt=2.5e-6)
     wz = O.random_vorticity(op, 1)
     if model == "ternary":
         reals = (wz,) + tuple(O.init_lens_real(op))
-    else:
+    elif model == "binary":
         reals = (wz,) + O.init_noise_real(op, 7, 0.5, 0.1)
+    else:
+        reals = (wz,)
     s0 = O.state_from_real(op, *reals)
     s1 = O.etd2rk(op, s0)
     with C.Context(cp) as ctx:
