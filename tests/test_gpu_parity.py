@@ -143,11 +143,11 @@ def test_larger_grids_10_steps(model, N):
         assert rel(a - a0, b - a0) < 1e-11
 
 
+@pytest.mark.parametrize("N", [1024, 2048])
 @pytest.mark.parametrize("model", ["ns", "binary", "ternary"])
-def test_full_size_one_step(model):
-    """2048^2 (one row per row-kernel workgroup; LDS-staged inputs for the phase
-    models, direct row loads for NS): one step vs the oracle."""
-    N = 2048
+def test_full_size_one_step(model, N):
+    """1024^2 (two rows per row-kernel workgroup) and 2048^2 (one row; LDS-staged
+    inputs for the phase models, direct row loads for NS): one step vs the oracle."""
     op, cp, _ = case(model, N, nu=2e-3, lam=5e-4, dt=2.5e-6)
     wz = O.random_vorticity(op, 1)
     if model == "ternary":
