@@ -237,9 +237,10 @@ __device__ __forceinline__ void dma_copy16(const cd* src, unsigned lds) {
 // Streams the dealiased rows of the next field pair into LDS while the
 // current pair is transformed (one row per workgroup, T threads): lane l of a
 // wave copies one 16-byte entry into slot base + l of the staging area.
-template <int LOG2N>
+// JM: the longest row, N/4 + 1 kept modes (N/2 + 1 with the smooth filter).
+template <int LOG2N, int JM = (1 << LOG2N) / 4 + 1>
 struct RowStager {
-  static constexpr int N = 1 << LOG2N, T = N / 8, JMAX = N / 4 + 1;
+  static constexpr int N = 1 << LOG2N, T = N / 8, JMAX = JM;
   cd* stg;   // 2*JMAX slots: X in [0, Jt), Y in [Jt, 2 Jt)
   int Jt;
   template <bool HASY>
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
   const double sc = P.scale;
   cd v[8];
   RowTw<LOG2N, RC::R> tws;
-  tws.template init<RC::R == 1 && !FILT>(B.tw, t);   // the stager follows at one row per workgroup
+  tws.template init<RC::R == 1>(B.tw, t);   // the stager follows at one row per workgroup
   if constexpr (MODEL == MODEL_NS) {
     double u[8], w[8];
     // u, v then u.grad w (nonlin.c:47-48,62), cnlz = -F(nlz) (nonlin.c:74,92)
@@ -571,6 +572,13 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
     __shared__ cd stgA[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
     __shared__ cd stgB[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
     const RowStager<LOG2N> sa{stgA, P.Jt}, sb{stgB, P.Jt};
+    // Smooth filter: every mode is kept, so the rows are twice as long and a
+    // single buffer streams pair k+1 in while pair k is transformed (two
+    // buffers would leave one workgroup per CU).
+    constexpr bool STG1 = RC::R == 1 && FILT;
+    constexpr int JF = RC::N / 2 + 1;
+    __shared__ cd stgF[STG1 ? 2 * JF : 1];
+    const RowStager<LOG2N, JF> s1{stgF, P.Jt};
     double u[8], w[8], fx[8], fy[8], ad1[8], ad2[8];
     {
       double A1[8], A2[8];
@@ -581,6 +589,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         sa.template assemble<false, TER>(P.f, t, v);
         lds_sync();
         sa.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+      } else if constexpr (STG1) {
+        s1.template issue<TER>(B.Hmu1 + ro, B.Hmu2 + ro);
+        s1.wait();
+        s1.template assemble<false, TER>(P.f, t, v);
+        lds_sync();
+        s1.template issue<true>(B.Hu + ro, B.Hv + ro);
       } else {
         row_load_pair<LOG2N, false>(B.Hmu1 + ro, TER ? B.Hmu2 + ro : nullptr, P.Jt, P.f, t, v);
       }
@@ -601,6 +615,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         lds_sync();
         if (TER) sb.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
         else sb.template issue<true>(B.Hwx + ro, B.Hwy + ro);
+      } else if constexpr (STG1) {
+        s1.wait();
+        s1.template assemble<false, true>(P.f, t, v);
+        lds_sync();
+        s1.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
       } else {
         row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
       }
@@ -617,6 +636,12 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
           lds_sync();
           sa.template issue<true>(B.Hwx + ro, B.Hwy + ro);
         }
+      } else if constexpr (STG1) {
+        s1.wait();
+        s1.template assemble<true, true>(P.f, t, v);
+        lds_sync();
+        if (TER) s1.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
+        else s1.template issue<true>(B.Hwx + ro, B.Hwy + ro);
       } else {
         row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
       }
@@ -632,6 +657,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
         if constexpr (STG) {
           sb.template wait_older<true>();
           sb.template assemble<true, true>(P.f, t, v);
+        } else if constexpr (STG1) {
+          s1.wait();
+          s1.template assemble<true, true>(P.f, t, v);
+          lds_sync();
+          s1.template issue<true>(B.Hwx + ro, B.Hwy + ro);
         } else {
           row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
         }
@@ -673,6 +703,10 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
       sw.wait();
       rfft_split<LOG2N, RC::R, false>(v, lds, tws, t, put23);   // after the wait's barrier
       sw.template assemble<false, true>(P.f, t, v);
+    } else if constexpr (STG1) {
+      s1.wait();
+      rfft_split<LOG2N, RC::R, false>(v, lds, tws, t, put23);   // after the wait's barrier
+      s1.template assemble<false, true>(P.f, t, v);
     } else {
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, put23);
       row_load_pair<LOG2N, false>(B.Hwx + ro, B.Hwy + ro, P.Jt, P.f, t, v);
