@@ -158,15 +158,16 @@ int chns_ensemble_size(const chns_ensemble *e);
  * Its work runs on the ensemble's stream, ordered with chns_ensemble_step.
  * Owned by the ensemble: chns_destroy on it does nothing. */
 chns_ctx *chns_ensemble_instance(chns_ensemble *e, int b);
-/* nsteps ETD2RK steps of every instance (each instance needs a state).  One
- * step of the whole ensemble is captured into a HIP graph on first use and
- * replayed per step.  Returns when queued; chns_ensemble_sync waits. */
+/* nsteps ETD2RK steps of every instance (each instance needs a state).  Eight
+ * steps of the whole ensemble are captured into a HIP graph on first use and
+ * replayed per eight steps, the remainder launched directly.  Returns when
+ * queued; chns_ensemble_sync waits. */
 int chns_ensemble_step(chns_ensemble *e, int nsteps);
 int chns_ensemble_sync(chns_ensemble *e);
 /* Autotune the launch split of every group (1, 2 or 4 concurrent slices),
- * timing `reps` replayed steps per candidate, largest groups first; the
- * instances' states are restored afterwards.  splits (may be NULL) receives
- * the choice per group, in group order. */
+ * timing `reps` replays of the 8-step graph per candidate, largest groups
+ * first; the instances' states are restored afterwards.  splits (may be
+ * NULL) receives the choice per group, in group order. */
 int chns_ensemble_tune(chns_ensemble *e, int reps, int *splits);
 /* Groups in order: log2 N, model and instance count (arrays may be NULL);
  * returns the number of groups. */

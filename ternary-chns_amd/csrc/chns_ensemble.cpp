@@ -112,25 +112,33 @@ int step_part(chns_ensemble* e, const chns_ensemble::Group& g, const chns_ensemb
   return CHNS_OK;
 }
 
-// Capture one step of every group: s0 forks to the slice streams and joins back.
+// Steps per replay of the ensemble graph: as for chns_step, a graph of 8
+// steps replaces 8 replays of a one-step graph.
+constexpr int kGraphSteps = 8;
+
+// One step of every group: s0 forks to the slice streams and joins back
+// (launched directly, or recorded while s0 is captured).
+int enqueue_step(chns_ensemble* e) {
+  ENS_HIP(e, hipEventRecord(e->fork, e->s0));
+  for (auto& g : e->groups)
+    for (auto& p : g.parts) {
+      ENS_HIP(e, hipStreamWaitEvent(p.s, e->fork, 0));
+      const int rc = step_part(e, g, p);
+      if (rc != CHNS_OK) return rc;
+      ENS_HIP(e, hipEventRecord(p.done, p.s));
+      ENS_HIP(e, hipStreamWaitEvent(e->s0, p.done, 0));
+    }
+  return CHNS_OK;
+}
+
+// Capture kGraphSteps steps of every group.
 int build_graph(chns_ensemble* e) {
   drop_graph(e);
   ENS_HIP(e, hipStreamBeginCapture(e->s0, hipStreamCaptureModeThreadLocal));
   int rc = CHNS_OK;
-  hipError_t r = hipEventRecord(e->fork, e->s0);
-  for (auto& g : e->groups)
-    for (auto& p : g.parts) {
-      if (r == hipSuccess) r = hipStreamWaitEvent(p.s, e->fork, 0);
-      if (r == hipSuccess && rc == CHNS_OK) rc = step_part(e, g, p);
-      if (r == hipSuccess) r = hipEventRecord(p.done, p.s);
-      if (r == hipSuccess) r = hipStreamWaitEvent(e->s0, p.done, 0);
-    }
+  for (int k = 0; k < kGraphSteps && rc == CHNS_OK; ++k) rc = enqueue_step(e);
   hipGraph_t graph = nullptr;
-  hipError_t r2 = hipStreamEndCapture(e->s0, &graph);
-  if (r != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return ehip(e, r, "graph capture");
-  }
+  const hipError_t r2 = hipStreamEndCapture(e->s0, &graph);
   if (rc != CHNS_OK || r2 != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
     return rc != CHNS_OK ? rc : ehip(e, r2, "hipStreamEndCapture");
@@ -153,8 +161,14 @@ int refresh_inputs(chns_ensemble* e) {
   return CHNS_OK;
 }
 
+// nsteps steps: kGraphSteps per graph replay, the remainder launched directly.
 int replay(chns_ensemble* e, int nsteps) {
-  for (int n = 0; n < nsteps; ++n) ENS_HIP(e, hipGraphLaunch(e->exec, e->s0));
+  int n = 0;
+  for (; n + kGraphSteps <= nsteps; n += kGraphSteps) ENS_HIP(e, hipGraphLaunch(e->exec, e->s0));
+  for (; n < nsteps; ++n) {
+    const int rc = enqueue_step(e);
+    if (rc != CHNS_OK) return rc;
+  }
   return CHNS_OK;
 }
 
@@ -273,9 +287,9 @@ extern "C" int chns_ensemble_sync(chns_ensemble* e) {
 }
 
 extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
-  // Greedy per group, largest grids first: time `reps` graph replays for each
-  // split of the group into 1, 2, 4 concurrently launched slices and keep the
-  // fastest.  The instances' spectral state is saved before and restored
+  // Greedy per group, largest grids first: time `reps` replays of the graph
+  // (kGraphSteps steps each) for each split of the group into 1, 2, 4
+  // concurrently launched slices and keep the fastest.  The instances' spectral state is saved before and restored
   // after, so tuning leaves the physics untouched.
   if (!e || reps < 1) return CHNS_E_ARG;
   ENS_HIP(e, hipSetDevice(e->dev));
@@ -313,12 +327,13 @@ extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
   auto time_graph = [&](double* ms) -> int {
     int r = build_graph(e);
     if (r != CHNS_OK) return r;
-    if ((r = replay(e, 1)) != CHNS_OK) return r;   // warm-up
+    if ((r = replay(e, kGraphSteps)) != CHNS_OK) return r;   // warm-up: one graph replay
     ENS_HIP(e, hipStreamSynchronize(e->s0));
     const auto t0 = std::chrono::steady_clock::now();
-    if ((r = replay(e, reps)) != CHNS_OK) return r;
+    if ((r = replay(e, reps * kGraphSteps)) != CHNS_OK) return r;   // reps graph replays
     ENS_HIP(e, hipStreamSynchronize(e->s0));
-    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+          (reps * kGraphSteps);
     return CHNS_OK;
   };
   for (int gi : order) {

@@ -80,19 +80,19 @@ def test_ensemble_matches_single_contexts_across_replays():
     with C.Ensemble(ps) as ens:
         for b, s in enumerate(states):
             ens[b].set_state(*s.fields())
-        ens.step(3)
+        ens.step(9)                           # one 8-step graph replay and one direct step
         ke = ens[0].kinetic_energy()          # eager work between graph replays
         ens[2].truncate_state()               # spectrum() side effect on one instance
-        ens.step(4)
+        ens.step(8)
         ens.sync()
         got = [ens[b].get_state() for b in range(len(MEMBERS))]
     for b, (p, s) in enumerate(zip(ps, states)):
-        ref = single(p, s, 7, truncate_after=3 if b == 2 else None)
+        ref = single(p, s, 17, truncate_after=9 if b == 2 else None)
         assert close(got[b], ref, TOL_SAME), b
-    # member 0 against the oracle after 3 steps (kinetic energy) and 7 steps (state)
-    s3 = O.run(ops[0], states[0], 3)
-    assert abs(ke[0] - O.kinetic_energy(ops[0], s3)[0]) <= 1e-12 * abs(ke[0])
-    assert close(got[0], O.run(ops[0], states[0], 7).fields(), 1e-10)
+    # member 0 against the oracle after 9 steps (kinetic energy) and 17 steps (state)
+    s9 = O.run(ops[0], states[0], 9)
+    assert abs(ke[0] - O.kinetic_energy(ops[0], s9)[0]) <= 1e-12 * abs(ke[0])
+    assert close(got[0], O.run(ops[0], states[0], 17).fields(), 1e-10)
 
 
 def test_state_upload_between_steps():
@@ -100,12 +100,12 @@ def test_state_upload_between_steps():
     with C.Ensemble(ps) as ens:
         for b, s in enumerate(states):
             ens[b].set_state(*s.fields())
-        ens.step(2)
+        ens.step(8)
         ens[1].set_state(*states[1].fields())   # restart instance 1: its row inputs are rebuilt
-        ens.step(2)
+        ens.step(10)
         got = [ens[b].get_state() for b in range(4)]
     for b in range(4):
-        ref = single(ps[b], states[b], 2 if b == 1 else 4)
+        ref = single(ps[b], states[b], 10 if b == 1 else 18)
         assert close(got[b], ref, TOL_SAME), b
 
 
@@ -114,17 +114,17 @@ def test_tune_restores_state_and_keeps_results():
     with C.Ensemble(ps) as ens:
         for b, s in enumerate(states):
             ens[b].set_state(*s.fields())
-        ens.step(2)
+        ens.step(8)
         before = [ens[b].get_state() for b in range(len(MEMBERS))]
         splits = ens.tune(reps=2)
         assert len(splits) == len(ens.groups()) and all(s in (1, 2, 4) for s in splits)
         after = [ens[b].get_state() for b in range(len(MEMBERS))]
-        ens.step(3)
+        ens.step(11)
         got = [ens[b].get_state() for b in range(len(MEMBERS))]
     for a, b in zip(before, after):
         assert close(a, b, 0.0)
     for b, (p, s) in enumerate(zip(ps, states)):
-        assert close(got[b], single(p, s, 5), TOL_SAME), b
+        assert close(got[b], single(p, s, 19), TOL_SAME), b
 
 
 def test_ensemble_needs_every_state():
