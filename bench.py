@@ -227,7 +227,7 @@ def ensemble_bench(steps: int, warmup: int):
         "dtype": "f64",
         "data": "synthetic (SplitMix64 noise, seeds 1000+b / 2000+b)",
         "config": {"workload": "ensemble 32x256^2 + 16x512^2 + 4x1024^2, binary/ternary alternating, "
-                               "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3 (ternary x (min|gamma|/0.1)^2), one HIP graph replay per step",
+                               "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3 (ternary x (min|gamma|/0.1)^2), HIP graph replays of 8 steps",
                    "instances": len(members), "groups": groups, "parallelism": "single GPU"},
         "sequential_s": seq,
         "batch_s": {"untuned": res[False][0], "tuned": res[True][0]},
