@@ -25,6 +25,7 @@ extern "C" {
 #define CHNS_E_ARG (-1)     /* invalid argument or unsupported configuration */
 #define CHNS_E_HIP (-2)     /* HIP runtime error */
 #define CHNS_E_STATE (-3)   /* call out of order (e.g. no state set) */
+#define CHNS_E_DIVERGED (-4) /* chns_status: the state went non-finite or the free-energy guard tripped */
 
 /* models: macro.h:7-8 (neither BINARY nor TERNARY = Navier-Stokes only) */
 #define CHNS_MODEL_NS 0
@@ -41,7 +42,14 @@ extern "C" {
  * chi(x) of init.c:184-206 with eta = 1.5 dt (global.h:47-48) acting as in
  * nonlin.c:63-95,209-211,294-296,337-339 and surface_tension.c:83-85,
  * 151-154,202-205; sponge - the sponge(y)/eta damping of the vorticity
- * (init.c:207-211, nonlin.c:50-99), which needs vpm. */
+ * (init.c:207-211, nonlin.c:50-99), which needs vpm.
+ * dealias: the truncation() rule.  0 (default) is the reference's circular
+ * cut, zero where kx^2 + ky^2 > ((Nx/4) 2pi/lx)^2 (functions.c:56-59,75-78);
+ * 1 is the 2/3 rule, zero where |kx| > (Nx/3) 2pi/lx or |ky| > (Ny/3) 2pi/ly;
+ * 2 is the circular cut at Nx/3 instead of Nx/4 (the Nx/3 radius of
+ * energy.c:13-15).  Rules 1 and 2 run the unpruned kernels of the smooth
+ * filter with a 0/1 factor table; single contexts only, not with
+ * smooth_filter (which replaces the cut). */
 typedef struct {
   int Nx, Ny, model;
   double lx, ly, nu, dt;
@@ -55,6 +63,7 @@ typedef struct {
   double rho1, rho2, rho3, g;
   int vpm;           /* macro.h VPM */
   int sponge;        /* macro.h SPONGE (requires vpm) */
+  int dealias;       /* truncation rule: 0 reference N/4 circle, 1 2/3 rule (square), 2 N/3 circle */
 } chns_params;
 
 typedef struct chns_ctx chns_ctx;
@@ -86,16 +95,38 @@ int chns_set_real_state(chns_ctx *ctx, const double *wz, const double *phi1, con
  * get_velocity() (velocity.c:3-44), curl() (curl.c:16-31), truncation()
  * (functions.c:47-83) and the dfft/ifft pairs (fft.c:11-31) inside them.
  * Returns when the work is queued; chns_sync() waits.  Steps run 8 at a
- * time as replays of a HIP graph of 8 captured steps (captured on the first
- * call with nsteps >= 8), the remainder as direct launches; with event
- * timing on (chns_timing_enable), or while the caller captures the
- * context's stream into a graph of its own, every kernel is launched
- * directly.  Both paths compute bit-identical states. */
+ * time as replays of a HIP graph of 8 captured steps (captured and uploaded
+ * on the first call, whatever its nsteps), the remainder as direct launches;
+ * with event timing on (chns_timing_enable), with graphs off
+ * (chns_graphs_enable), or while the caller captures the context's stream
+ * into a graph of its own, every kernel is launched directly.  Both paths
+ * compute bit-identical states. */
 int chns_step(chns_ctx *ctx, int nsteps);
 int chns_sync(chns_ctx *ctx);
 
+/* HIP graph replays of chns_step on (1, default) or off (0: every kernel
+ * launched directly, e.g. for profilers that trace launches one by one).
+ * The environment variable CHNS_NO_GRAPH=1 makes 0 the default of every
+ * context and ensemble created while it is set.  No reference counterpart. */
+int chns_graphs_enable(chns_ctx *ctx, int enable);
+
+/* Divergence status.  The reference's only guard is free_energy.c:151-154,
+ * MPI_Abort when the bulk free energy ephi >= 100; this library never
+ * aborts and reports instead.  Waits for the context's queued work, then
+ * returns CHNS_E_DIVERGED if a step since the state was last set wrote a
+ * non-finite value into it (a flag raised on the device by the ETD2RK
+ * update, CHNS_FLAG_NONFINITE), or if the last chns_free_energy found
+ * ephi >= 100 or non-finite (CHNS_FLAG_ENERGY); CHNS_OK otherwise.  flags
+ * (may be NULL) receives the bits.  chns_set_state and chns_set_real_state
+ * clear both. */
+#define CHNS_FLAG_NONFINITE 1
+#define CHNS_FLAG_ENERGY 2
+int chns_status(chns_ctx *ctx, int *flags);
+
 /* In-place truncation of the phase fields, the state side effect of
- * spectrum() (spectrum.c:22-27) called at main.c:226,281-284. */
+ * spectrum() (spectrum.c:22-27) called at main.c:226,281-284.  The next
+ * chns_step re-reads the truncated fields (with SMOOTH_FILTER truncation is
+ * not idempotent, so the stage inputs are rebuilt from the new state). */
 int chns_truncate_state(chns_ctx *ctx);
 
 /* ---- diagnostics (device computation, fixed-order reductions) ---------- */
@@ -178,7 +209,8 @@ int chns_ensemble_groups(const chns_ensemble *e, int *log2n, int *model, int *co
  * etd2rk.c:14-224 with everything it calls) ------------------------------
  *
  * Usage: set the initial state, chns_adj_forward (the state ends at the final
- * time), seed dL/d(final state) with chns_adj_seed_spectrum or chns_adj_seed,
+ * time), seed dL/d(final state) with chns_adj_seed_spectrum,
+ * chns_adj_seed_free_energy or chns_adj_seed,
  * then chns_adj_backward.  Gradients are with respect to the real fields that
  * chns_set_real_state turns into the initial state, and to sigma_12/13/23.
  * Spectral adjoints follow one convention: Xbar is the unnormalised r2c of
@@ -223,7 +255,8 @@ int chns_etd_tables(const chns_params *p, double *eta, double *z1w, double *z2w,
                     double *beta, double *z1c, double *z2c);
 
 /* Dealias extent per ky column j < Ny/2+1: the largest |kx index| kept by
- * truncation() (functions.c:47-83), -1 if the column is fully truncated. */
+ * truncation() (functions.c:47-83) under p->dealias, -1 if the column is
+ * fully truncated. */
 int chns_dealias_extent(const chns_params *p, int *amax);
 
 /* SMOOTH_FILTER factor exp(-36 pp^18), pp = ksqr/ksqrmax, per mode in IDXC

@@ -67,6 +67,7 @@ class Params:
     sigma_13: float = 0.2
     Xlambda: float = 2.0
     smooth_filter: bool = False   # macro.h:15 SMOOTH_FILTER
+    dealias: int = 0              # truncation rule: 0 the reference's N/4 circle, 1 the 2/3 rule, 2 an N/3 circle
     f_amp: float = 0.0            # Kolmogorov forcing amplitude on omega
     f_k: int = 0                  # forcing wavenumber index along y
     gravity: bool = False         # macro.h GRAVITY (nonlin.c:107-171)
@@ -154,8 +155,19 @@ def ksqr_grid(p: Params):
 
 
 def dealias_mask(p: Params):
-    """Keep-mask of truncation() (functions.c:47-83): zero where ksqr > dax*dax."""
-    dax = (p.Nx / 4.0) * p.factor_x                    # functions.c:56
+    """Keep-mask of truncation() (functions.c:47-83): zero where ksqr > dax*dax.
+
+    ``p.dealias`` selects the rule.  0 is the reference's circle of radius
+    Nx/4 (functions.c:56-59).  1 is the 2/3 rule: zero where
+    |kx| > (Nx/3) factor_x or |ky| > (Ny/3) factor_y.  2 is the reference
+    predicate with radius Nx/3 (the Nx/3 of energy.c:13-15) instead of Nx/4.
+    """
+    if p.dealias == 1:
+        kx, ky = kxky(p)
+        dax = (p.Nx / 3.0) * p.factor_x
+        day = (p.Ny / 3.0) * p.factor_y
+        return ~(np.abs(kx)[:, None] > dax) & ~(ky[None, :] > day)
+    dax = (p.Nx / (3.0 if p.dealias == 2 else 4.0)) * p.factor_x   # functions.c:56
     dealias = dax * dax                                # functions.c:59
     return ~(ksqr_grid(p) > dealias)                   # functions.c:75-78
 

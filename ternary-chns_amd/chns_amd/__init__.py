@@ -24,7 +24,8 @@ LIB_PATH = os.path.join(_HERE, "libchns_amd.so")
 MODEL_NS, MODEL_BINARY, MODEL_TERNARY = 0, 1, 2
 _MODELS = {"ns": MODEL_NS, "binary": MODEL_BINARY, "ternary": MODEL_TERNARY}
 
-CHNS_OK, CHNS_E_ARG, CHNS_E_HIP, CHNS_E_STATE = 0, -1, -2, -3
+CHNS_OK, CHNS_E_ARG, CHNS_E_HIP, CHNS_E_STATE, CHNS_E_DIVERGED = 0, -1, -2, -3, -4
+CHNS_FLAG_NONFINITE, CHNS_FLAG_ENERGY = 1, 2
 
 EXPORTS = (
     "chns_params_default", "chns_create", "chns_destroy", "chns_last_error", "chns_set_state",
@@ -34,7 +35,8 @@ EXPORTS = (
     "chns_diag_sizes", "chns_ensemble_create", "chns_ensemble_destroy", "chns_ensemble_last_error",
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
-    "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store",
+    "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store", "chns_graphs_enable",
+    "chns_status",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -56,6 +58,7 @@ class CParams(ctypes.Structure):
         ("gravity", ctypes.c_int),
         ("rho1", ctypes.c_double), ("rho2", ctypes.c_double), ("rho3", ctypes.c_double), ("g", ctypes.c_double),
         ("vpm", ctypes.c_int), ("sponge", ctypes.c_int),
+        ("dealias", ctypes.c_int),
     ]
 
 
@@ -119,6 +122,8 @@ def load_library(path: str = LIB_PATH):
     lib.chns_adj_get.argtypes = [V, D, D, D, D]
     lib.chns_adj_release.argtypes = [V]
     lib.chns_adj_set_store.argtypes = [V, I]
+    lib.chns_graphs_enable.argtypes = [V, I]
+    lib.chns_status.argtypes = [V, ctypes.POINTER(I)]
     _lib = lib
     return lib
 
@@ -228,6 +233,20 @@ class Context:
 
     def truncate_state(self):
         self._check(self.lib.chns_truncate_state(self.h), "chns_truncate_state")
+
+    def graphs(self, enable: bool = True):
+        """HIP graph replays of step() on (default) or off (every kernel launched directly)."""
+        self._check(self.lib.chns_graphs_enable(self.h, int(bool(enable))), "chns_graphs_enable")
+
+    def status(self):
+        """Divergence flags (CHNS_FLAG_NONFINITE | CHNS_FLAG_ENERGY), 0 when healthy; waits
+        for the queued work.  Does not raise on divergence: the caller decides, as the
+        reference's MPI_Abort at free_energy.c:151-154 did."""
+        f = ctypes.c_int()
+        rc = self.lib.chns_status(self.h, ctypes.byref(f))
+        if rc not in (CHNS_OK, CHNS_E_DIVERGED):
+            self._check(rc, "chns_status")
+        return f.value
 
     # -- diagnostics (free_energy.c, kinetic_energy.c, perimeter.c, spectrum.c)
     def free_energy(self):

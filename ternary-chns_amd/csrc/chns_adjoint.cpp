@@ -215,9 +215,10 @@ int check_ctx(chns_ctx* c) {
   if (!c) return CHNS_E_ARG;
   if (c->p.model != CHNS_MODEL_TERNARY) return chns_fail(c, CHNS_E_ARG, "the adjoint supports the ternary model");
   if (c->in_ensemble) return chns_fail(c, CHNS_E_ARG, "the adjoint runs on a stand-alone context");
-  if (c->p.gravity || c->p.vpm || c->p.sponge || c->p.smooth_filter || c->p.f_amp != 0.0)
+  if (c->p.gravity || c->p.vpm || c->p.sponge || c->p.smooth_filter || c->p.dealias != 0 || c->p.f_amp != 0.0)
     return chns_fail(c, CHNS_E_ARG,
-                     "the adjoint covers the plain ternary step (no gravity, vpm, sponge, smooth_filter, forcing)");
+                     "the adjoint covers the plain ternary step (no gravity, vpm, sponge, smooth_filter, dealias rule "
+                     "other than 0, forcing)");
   AHIP(c, hipSetDevice(c->dev));
   if (!c->adj) c->adj = new chns_adj_state();
   return ensure_work(c);

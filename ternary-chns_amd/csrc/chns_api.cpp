@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +22,11 @@ int chns_fail(chns_ctx* c, int code, const std::string& msg) {
 
 int chns_hip_fail(chns_ctx* c, hipError_t e, const char* what) {
   return chns_fail(c, CHNS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool chns_graphs_default() {
+  const char* v = std::getenv("CHNS_NO_GRAPH");
+  return !(v && v[0] == '1');
 }
 
 int chns_dalloc_bytes(chns_ctx* c, void** p, size_t bytes) {
@@ -156,6 +162,7 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   chns_ctx* c = new chns_ctx();
   c->p = *p;
   c->dev = device;
+  c->graphs = chns_graphs_default();
   chns_host::derive(p, &c->d);
   if (c->p.epsilon <= 0.0) c->p.epsilon = c->d.eps;
   hipError_t e = hipSetDevice(device);
@@ -229,15 +236,19 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
     b.chi = dchi;
     b.spg = dspg;
   }
-  if (p->smooth_filter) {
+  if (p->smooth_filter || p->dealias != 0) {
+    // the smooth filter's factors, or 1/0 on the kept/cut modes of the 2/3
+    // rule and the N/3 circle: both run the unpruned (table) kernels
     std::vector<double> ft(HH);
-    chns_host::build_filter_table(p, ft.data());
+    if (p->smooth_filter) chns_host::build_filter_table(p, ft.data());
+    else chns_host::build_cut_table(p, c->amask.data(), ft.data());
     double* dft;
     if ((rc = dalloc(c, &dft, HH))) return bail(rc);
     if ((e = hipMemcpy(dft, ft.data(), HH * 8, hipMemcpyHostToDevice)) != hipSuccess)
       return bail(hip_fail(c, e, "hipMemcpy(filter)"));
     b.filt = dft;
   }
+  if ((rc = dalloc(c, &b.flag, 1))) return bail(rc);
   if (p->gravity && (rc = dalloc(c, &b.GR, S))) return bail(rc);
   if (p->sponge && (rc = dalloc(c, &b.Hw, RJ))) return bail(rc);
   b.tw = dtw;
@@ -274,9 +285,11 @@ extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi
     CHNS_HIP(c, hipMemcpyAsync(c->dstage, src[f], bytes, hipMemcpyHostToDevice, c->stream));
     CHNS_LAUNCH(c, chns::launch_transpose(c->dstage, dst[f], N, H, c->stream));
   }
+  CHNS_HIP(c, hipMemsetAsync(c->b.flag, 0, sizeof(int), c->stream));
   CHNS_HIP(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
   c->hc_valid = false;
+  c->energy_flag = 0;
   return CHNS_OK;
 }
 
@@ -330,8 +343,11 @@ extern "C" int chns_set_real_state(chns_ctx* c, const double* wz, const double* 
   }
   if (c->nph == 1 && (rc = dfft_trunc(phi1, nullptr, c->b.C1, nullptr))) return rc;
   if (c->nph == 2 && (rc = dfft_trunc(phi1, phi2, c->b.C1, c->b.C2))) return rc;
+  CHNS_HIP(c, hipMemsetAsync(c->b.flag, 0, sizeof(int), c->stream));
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
   c->have_state = true;
   c->hc_valid = false;
+  c->energy_flag = 0;
   return CHNS_OK;
 }
 
@@ -374,7 +390,7 @@ int enqueue_step(chns_ctx* c) {
 constexpr int kGraphSteps = 8;
 
 // Capture kGraphSteps steps into c->step_exec (kernel nodes only: no memset
-// or copy).
+// or copy) and upload it, so its first replay starts like the later ones.
 int capture_step(chns_ctx* c) {
   CHNS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = CHNS_OK;
@@ -393,15 +409,17 @@ int capture_step(chns_ctx* c) {
   }
   c->step_graph = graph;
   c->step_exec = exec;
+  CHNS_HIP(c, hipGraphUpload(exec, c->stream));
   return CHNS_OK;
 }
 
 }  // namespace
 
 // Steps are replayed kGraphSteps at a time from a graph of captured steps,
-// the remainder launched directly; with event timing on, or when the caller
-// is capturing the stream into a graph of its own, every pass is launched
-// directly.
+// the remainder launched directly; the graph is captured on the first call
+// whatever its length, so a short first call (a warm-up) leaves it ready.
+// With event timing on, graphs off, or when the caller is capturing the
+// stream into a graph of its own, every pass is launched directly.
 extern "C" int chns_step(chns_ctx* c, int nsteps) {
   if (!c) return CHNS_E_ARG;
   if (nsteps < 0) return fail(c, CHNS_E_ARG, "nsteps < 0");
@@ -417,14 +435,14 @@ extern "C" int chns_step(chns_ctx* c, int nsteps) {
     }));
   hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
   CHNS_HIP(c, hipStreamIsCapturing(c->stream, &cap));
-  if (c->timing || cap != hipStreamCaptureStatusNone) {
+  if (c->timing || !c->graphs || cap != hipStreamCaptureStatusNone) {
     for (int n = 0; n < nsteps; ++n) {
       const int rc = enqueue_step(c);
       if (rc != CHNS_OK) return rc;
     }
   } else {
     int n = 0;
-    if (nsteps >= kGraphSteps && !c->step_exec) {
+    if (!c->step_exec) {
       const int rc = capture_step(c);
       if (rc != CHNS_OK) return rc;
     }
@@ -451,7 +469,32 @@ extern "C" int chns_truncate_state(chns_ctx* c) {
   CHNS_HIP(c, hipSetDevice(c->dev));
   if (c->nph >= 1) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C1, c->b.amask, c->b.filt, c->d.N, c->stream));
   if (c->nph >= 2) CHNS_LAUNCH(c, chns::launch_truncate(c->b.C2, c->b.amask, c->b.filt, c->d.N, c->stream));
+  // Pass A's outputs (the column iFFTs of T(c_i)) were made from the state
+  // before this truncation: the next step rebuilds them from T(T(c_i)).  With
+  // the sharp cut T is idempotent and they would not change; with the smooth
+  // filter T(T(c)) multiplies by the factor twice more (spectrum.c:24-26, then
+  // chem_potential.c:27-32 at the next step).
+  if (c->nph >= 1) c->hc_valid = false;
   return CHNS_OK;
+}
+
+extern "C" int chns_graphs_enable(chns_ctx* c, int enable) {
+  if (!c) return CHNS_E_ARG;
+  c->graphs = enable != 0;
+  return CHNS_OK;
+}
+
+extern "C" int chns_status(chns_ctx* c, int* flags) {
+  if (!c) return CHNS_E_ARG;
+  CHNS_HIP(c, hipSetDevice(c->dev));
+  int dev_flag = 0;
+  CHNS_HIP(c, hipMemcpyAsync(&dev_flag, c->b.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  CHNS_HIP(c, hipStreamSynchronize(c->stream));
+  const int f = (dev_flag ? CHNS_FLAG_NONFINITE : 0) | (c->energy_flag ? CHNS_FLAG_ENERGY : 0);
+  if (flags) *flags = f;
+  if (f == 0) return CHNS_OK;
+  return fail(c, CHNS_E_DIVERGED, (f & CHNS_FLAG_NONFINITE) ? "a step wrote a non-finite value into the state"
+                                                            : "bulk free energy ephi >= 100 (free_energy.c:151-154)");
 }
 
 extern "C" int chns_timing_enable(chns_ctx* c, int enable) {

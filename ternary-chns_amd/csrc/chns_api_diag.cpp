@@ -86,6 +86,7 @@ extern "C" int chns_free_energy(chns_ctx* c, double* ephi, double* egphi) {
   const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);   // free_energy.c:211
   const double eps = c->d.eps;
   double q[chns::RED_Q];
+  double e_bulk = 0.0, e_grad = 0.0;
   if (c->nph == 2) {
     // untruncated c1, c2 and their gradients (free_energy.c:218-221, 301-313)
     if ((rc = real_pair(c, 1, chns::COP_ID, 2, chns::COP_ID, false, 0, 1)) ||
@@ -94,17 +95,21 @@ extern "C" int chns_free_energy(chns_ctx* c, double* ephi, double* egphi) {
       return rc;
     const double* f[6] = {real_buf(c, 0), real_buf(c, 1), real_buf(c, 2), real_buf(c, 3), real_buf(c, 4), real_buf(c, 5)};
     if ((rc = reduce(c, chns::RED_FE_TER, f, q))) return rc;
-    if (ephi) *ephi = q[0] * (6.0 / eps) * scale;              // free_energy.c:246-248
-    if (egphi) *egphi = q[1] * (0.375 * eps * scale);           // free_energy.c:335
+    e_bulk = q[0] * (6.0 / eps) * scale;                        // free_energy.c:246-248
+    e_grad = q[1] * (0.375 * eps * scale);                      // free_energy.c:335
   } else {
     if ((rc = real_pair(c, 1, chns::COP_ID, 1, chns::COP_IKX, false, 0, 1)) ||
         (rc = real_pair(c, 1, chns::COP_IKY, -1, 0, false, 2, -1)))
       return rc;
     const double* f[6] = {real_buf(c, 0), real_buf(c, 1), real_buf(c, 2), nullptr, nullptr, nullptr};
     if ((rc = reduce(c, chns::RED_FE_BIN, f, q))) return rc;
-    if (ephi) *ephi = q[0] * (12.0 * (c->p.sigma / eps)) * scale;       // free_energy.c:233-236
-    if (egphi) *egphi = q[1] * ((0.75 * c->p.sigma * eps) * scale);     // free_energy.c:290
+    e_bulk = q[0] * (12.0 * (c->p.sigma / eps)) * scale;                // free_energy.c:233-236
+    e_grad = q[1] * ((0.75 * c->p.sigma * eps) * scale);                // free_energy.c:290
   }
+  if (ephi) *ephi = e_bulk;
+  if (egphi) *egphi = e_grad;
+  // the reference aborts the run here (free_energy.c:151-154); chns_status reports it
+  c->energy_flag = !(e_bulk < 100.0);
   return CHNS_OK;
 }
 

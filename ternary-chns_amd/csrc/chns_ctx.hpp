@@ -27,6 +27,10 @@ struct chns_ctx {
   // are fixed when the context is created, so the graph stays valid)
   hipGraph_t step_graph = nullptr;
   hipGraphExec_t step_exec = nullptr;
+  bool graphs = true;                        // chns_graphs_enable / CHNS_NO_GRAPH
+  // divergence status (chns_status): the device flag b.flag raised by pass E,
+  // and the host-side free-energy guard of the last chns_free_energy
+  int energy_flag = 0;
   // event timing (chns_timing_enable)
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -49,6 +53,8 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
 // c->dstage: one field in the reference layout IDXC [N][H], which
 // chns::launch_transpose turns into or out of the column-major [H][N] state
 int chns_stage_buffer(chns_ctx* c);
+// CHNS_NO_GRAPH=1 in the environment: graphs off by default
+bool chns_graphs_default();
 // packed real [N][N] c2r (scaled 1/(Nx*Ny)) of a column-major spectral field S
 // of this context, copied to host memory out (synchronises)
 int chns_real_of(chns_ctx* c, const chns::cd* S, double* out);

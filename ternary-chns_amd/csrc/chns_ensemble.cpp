@@ -42,6 +42,7 @@ struct chns_ensemble {
   std::vector<void*> allocs;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bool graphs = true;           // false (CHNS_NO_GRAPH=1): every step launched directly
   std::string err;
 };
 
@@ -145,6 +146,7 @@ int build_graph(chns_ensemble* e) {
   }
   e->graph = graph;
   ENS_HIP(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+  ENS_HIP(e, hipGraphUpload(e->exec, e->s0));
   return CHNS_OK;
 }
 
@@ -161,10 +163,17 @@ int refresh_inputs(chns_ensemble* e) {
   return CHNS_OK;
 }
 
-// nsteps steps: kGraphSteps per graph replay, the remainder launched directly.
+// nsteps steps: kGraphSteps per graph replay, the remainder launched directly
+// (every step, with graphs off).
 int replay(chns_ensemble* e, int nsteps) {
   int n = 0;
-  for (; n + kGraphSteps <= nsteps; n += kGraphSteps) ENS_HIP(e, hipGraphLaunch(e->exec, e->s0));
+  if (e->graphs) {
+    if (!e->exec) {
+      const int rc = build_graph(e);
+      if (rc != CHNS_OK) return rc;
+    }
+    for (; n + kGraphSteps <= nsteps; n += kGraphSteps) ENS_HIP(e, hipGraphLaunch(e->exec, e->s0));
+  }
   for (; n < nsteps; ++n) {
     const int rc = enqueue_step(e);
     if (rc != CHNS_OK) return rc;
@@ -205,6 +214,7 @@ extern "C" int chns_ensemble_create(chns_ensemble** out, const chns_params* p, i
   }
   chns_ensemble* e = new chns_ensemble();
   e->dev = device;
+  e->graphs = chns_graphs_default();
   auto bail = [&](int rc) {
     std::fprintf(stderr, "chns_ensemble_create: %s\n", e->err.c_str());
     chns_ensemble_destroy(e);
@@ -215,9 +225,10 @@ extern "C" int chns_ensemble_create(chns_ensemble** out, const chns_params* p, i
     return bail(ehip(e, r, "hipEventCreate"));
   std::map<std::pair<int, int>, int> gid;   // (log2n, model) -> group
   for (int b = 0; b < n; ++b) {
-    if (p[b].gravity || p[b].vpm || p[b].sponge || p[b].smooth_filter)
+    if (p[b].gravity || p[b].vpm || p[b].sponge || p[b].smooth_filter || p[b].dealias != 0)
       return bail(efail(e, CHNS_E_ARG, "instance " + std::to_string(b) +
-                                           ": the optional physics (gravity, vpm, sponge, smooth_filter) run on single contexts"));
+                                           ": the optional physics (gravity, vpm, sponge, smooth_filter) and the "
+                                           "dealias rules other than 0 run on single contexts"));
     chns_ctx* c = nullptr;
     const int rc = chns_create_on(&c, &p[b], device, e->s0);
     if (rc != CHNS_OK) return bail(efail(e, rc, "instance " + std::to_string(b) + ": invalid parameters"));
@@ -273,9 +284,8 @@ extern "C" int chns_ensemble_step(chns_ensemble* e, int nsteps) {
   if (nsteps < 0) return efail(e, CHNS_E_ARG, "nsteps < 0");
   if (nsteps == 0) return CHNS_OK;
   ENS_HIP(e, hipSetDevice(e->dev));
-  int rc = refresh_inputs(e);
+  const int rc = refresh_inputs(e);
   if (rc != CHNS_OK) return rc;
-  if (!e->exec && (rc = build_graph(e)) != CHNS_OK) return rc;
   return replay(e, nsteps);
 }
 
@@ -325,8 +335,8 @@ extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
     return (double)A.members.size() * (1 << (2 * A.log2n)) > (double)B.members.size() * (1 << (2 * B.log2n));
   });
   auto time_graph = [&](double* ms) -> int {
-    int r = build_graph(e);
-    if (r != CHNS_OK) return r;
+    drop_graph(e);   // the slices changed: replay() captures them anew
+    int r;
     if ((r = replay(e, kGraphSteps)) != CHNS_OK) return r;   // warm-up: one graph replay
     ENS_HIP(e, hipStreamSynchronize(e->s0));
     const auto t0 = std::chrono::steady_clock::now();

@@ -24,5 +24,8 @@ void build_wall_tables(const chns_params* p, double* chi, double* sponge);
 // SMOOTH_FILTER factor exp(-36 pp^18) of truncation() (functions.c:54,70-73)
 // at (j, |i|): ft[j*H + |i|], H = Ny/2+1
 void build_filter_table(const chns_params* p, double* ft);
+// 1 on the modes the cut keeps (|i| <= amax[j]), 0 elsewhere, in the layout of
+// build_filter_table: the factor table of the 2/3-rule and N/3 cuts
+void build_cut_table(const chns_params* p, const int* amax, double* ft);
 
 }  // namespace chns_host

@@ -798,6 +798,9 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
   cd* Nsv = g == 0 ? B.NW : (g == 1 ? B.NC1 : B.NC2);
   // Stage 1 keeps b = a - z2 N(u) instead of N(u), so stage 2 forms
   // u' = a + z2 (N(a) - N(u)) as b + z2 N(a) without reading a back.
+  // fin: every updated mode of this thread is finite (re + im is finite
+  // unless one part is inf or NaN, or both overflow together)
+  bool fin = true;
   if (ok) {
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
@@ -813,13 +816,16 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
         So[idx] = a;
         Nsv[idx] = csub(a, cscale(v[m], z2));
         s[m] = a;
+        fin = fin && __builtin_isfinite(a.x + a.y);
       } else {
         const cd a = cadd(Nsv[idx], cscale(v[m], z2));
         So[idx] = a;
         s[m] = a;
+        fin = fin && __builtin_isfinite(a.x + a.y);
       }
     }
   }
+  if (!fin) *B.flag = 1;   // a plain store of the same value from any lane (read by chns_status)
   // next stage's pass A for the phase fields
   if (g >= 1 && tile * CF::C < P.Jt) {
     const bool okA = j < P.Jt;

@@ -51,6 +51,7 @@ extern "C" void chns_params_default(chns_params* p, int model) {
   p->g = 3.0;
   p->vpm = 0;
   p->sponge = 0;
+  p->dealias = 0;             // functions.c:56-59: the circular N/4 cut
 }
 
 namespace chns_host {
@@ -74,6 +75,8 @@ const char* check_params(const chns_params* p) {
   if (p->gravity && p->model == CHNS_MODEL_NS) return "gravity needs a phase field (BINARY or TERNARY)";
   if (p->sponge && !p->vpm) return "sponge needs vpm (its profile is set up with the walls, init.c:184-211)";
   if ((p->vpm || p->sponge) && !(p->dt > 0.0)) return "vpm needs dt > 0 (eta = 1.5 dt)";
+  if (p->dealias < 0 || p->dealias > 2) return "dealias must be 0 (N/4 circle), 1 (2/3 rule) or 2 (N/3 circle)";
+  if (p->dealias != 0 && p->smooth_filter) return "smooth_filter replaces the cut: use dealias 0 with it";
   if (p->model == CHNS_MODEL_TERNARY) {
     Derived d;
     derive(p, &d);
@@ -206,7 +209,21 @@ void dealias_extent(const chns_params* p, int* amax) {
     for (int j = 0; j < d.H; ++j) amax[j] = p->Nx / 2;
     return;
   }
-  const double dax = (p->Nx / 4.0) * d.factor_x;      // functions.c:56
+  if (p->dealias == 1) {
+    // 2/3 rule: zero where |kx| > (Nx/3) 2pi/lx or |ky| > (Ny/3) 2pi/ly
+    const double dax = (p->Nx / 3.0) * d.factor_x, day = (p->Ny / 3.0) * d.factor_y;
+    for (int j = 0; j < d.H; ++j) {
+      int a = -1;
+      if (!(ky_of(d, j) > day))
+        for (int i = 0; i <= p->Nx / 2; ++i)
+          if (!(kx_of(p, d, i) > dax)) a = i;
+      amax[j] = a;
+    }
+    return;
+  }
+  // the reference's circular cut (functions.c:56-59, radius Nx/4), or the
+  // same predicate at radius Nx/3 (dealias 2)
+  const double dax = (p->Nx / (p->dealias == 2 ? 3.0 : 4.0)) * d.factor_x;   // functions.c:56
   const double dealias = dax * dax;                    // functions.c:59
   for (int j = 0; j < d.H; ++j) {
     int a = -1;
@@ -218,6 +235,12 @@ void dealias_extent(const chns_params* p, int* amax) {
     }
     amax[j] = a;
   }
+}
+
+void build_cut_table(const chns_params* p, const int* amax, double* ft) {
+  const int H = p->Ny / 2 + 1;
+  for (int j = 0; j < H; ++j)
+    for (int i = 0; i < H; ++i) ft[(size_t)j * H + i] = i <= amax[j] ? 1.0 : 0.0;
 }
 
 void build_filter_table(const chns_params* p, double* ft) {

@@ -47,12 +47,14 @@ struct Bufs {
   const double* tabw2;         // omega stage 2: z2
   const double* tabc2;         // phase stage 2: z2
   const int* amask;            // [Ny2] largest kept |i| per column, -1 if none
-  const double* filt;          // SMOOTH_FILTER: [(N/2+1)*(N/2+1)] factor exp(-36 pp^18) at j*(N/2+1) + |i|,
-                               // applied twice wherever truncation() is (null: the sharp cut)
+  const double* filt;          // [(N/2+1)*(N/2+1)] truncation factor at j*(N/2+1) + |i|, applied twice
+                               // wherever truncation() is: SMOOTH_FILTER's exp(-36 pp^18), or 1/0 for
+                               // the 2/3-rule and N/3 cuts (null: the pruned N/4 cut of the reference)
   const double* chi;           // [N] wall profile chi(x) (vpm)
   const double* spg;           // [N] sponge(y)/eta (sponge)
   cd* GR;                      // buoyancy torque, column-major (gravity)
   cd* Hw;                      // column iFFT of T(omega), TJ tiled (sponge)
+  int* flag;                   // divergence flag: set to 1 by pass E when an updated mode is not finite
 };
 
 // ---- diagnostics (chns_diag.hpp)

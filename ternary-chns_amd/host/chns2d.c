@@ -9,7 +9,8 @@
  * a truncated real dump every navg steps (DUMP_REAL_MPI, main.c:332-358).
  * Output files and printf formats are the reference's:
  *   data/ken_t.dat       kinetic_energy.c:109,135,141
- *   data/free_en_t.dat   free_energy.c:149 (abort when ephi >= 100, :151-153)
+ *   data/free_en_t.dat   free_energy.c:149 (exit 3 when chns_status reports ephi >= 100
+ *                        or a non-finite state, as free_energy.c:151-154 aborts)
  *   data/gamma.dat | data/gamma1.dat, data/gamma2.dat   perimeter.c:123,264-265
  *   data/volume_*.dat    opened and left empty, as main.c:123,161
  *   spectra/{spectrum,Xspectrum,Yspectrum}.N           spectrum.c:150-187
@@ -20,6 +21,7 @@
  *   sigma_12 sigma_13 sigma_23 Xlambda f_amp f_k
  *   gravity rho1 rho2 rho3 g vpm sponge   (macro.h GRAVITY / VPM / SPONGE switches)
  *   smooth_filter                         (macro.h SMOOTH_FILTER)
+ *   dealias=0|1|2  truncation rule: reference N/4 circle, 2/3 rule, N/3 circle
  *   init=noise|read|droplet|zero  nranks (noise: srand(10+2*rank) per slab)
  *   read1=path read2=path (raw float64 [Nx][Ny], INIT_READ_DATA; defaults
  *   init_lens/phi1.in, init_lens/phi2.in, binary init_field/phi.in)
@@ -104,6 +106,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "vpm")) p.vpm = atoi(v);           /* macro.h VPM */
     else if (!strcmp(k, "sponge")) p.sponge = atoi(v);     /* macro.h SPONGE */
     else if (!strcmp(k, "smooth_filter")) p.smooth_filter = atoi(v);   /* macro.h SMOOTH_FILTER */
+    else if (!strcmp(k, "dealias")) p.dealias = atoi(v);   /* truncation rule (0 = functions.c:56-59) */
     else if (!strcmp(k, "init")) init = v;
     else if (!strcmp(k, "nranks")) nranks = atoi(v);
     else if (!strcmp(k, "read1")) read1 = v;
@@ -197,7 +200,10 @@ int main(int argc, char **argv) {
         double ephi, egphi;                                                                         \
         CK(chns_free_energy(ctx, &ephi, &egphi), ctx);                                              \
         fprintf(fe, "%lf \t %.12lf \t %.12lf\n", t, ephi, egphi);                                   \
-        if (ephi >= 100) { fprintf(stderr, "chns2d: free energy diverged\n"); exit(3); }            \
+        if (chns_status(ctx, NULL) == CHNS_E_DIVERGED) {  /* free_energy.c:151-154 */              \
+          fprintf(stderr, "chns2d: %s\n", chns_last_error(ctx));                                    \
+          exit(3);                                                                                  \
+        }                                                                                           \
       }                                                                                             \
       CK(chns_kinetic_energy(ctx, &eu, &e1, &e2), ctx);                                             \
       if (nph == 2) fprintf(fu, "%lf \t %.12lf \t %.12lf \t %.12lf\n", t, eu, e1, e2);              \

@@ -49,18 +49,33 @@ def test_etd_tables_reference_defaults(model):
         assert np.array_equal(t[k].view(np.uint64), o[k].view(np.uint64)), k
 
 
+@pytest.mark.parametrize("rule", [0, 1, 2])
 @pytest.mark.parametrize("N,L", [(64, 2 * math.pi), (128, 4 * math.pi), (2048, 2 * math.pi), (256, 3.0)])
-def test_dealias_extent_matches_mask(N, L):
-    a = C.dealias_extent(C.params("ternary", Nx=N, Ny=N, lx=L, ly=L))
-    m = O.dealias_mask(O.Params(Nx=N, Ny=N, lx=L, ly=L))
+def test_dealias_extent_matches_mask(N, L, rule):
+    a = C.dealias_extent(C.params("ternary", Nx=N, Ny=N, lx=L, ly=L, dealias=rule))
+    m = O.dealias_mask(O.Params(Nx=N, Ny=N, lx=L, ly=L, dealias=rule))
     for j in range(N // 2 + 1):
         kept = np.where(m[:, j])[0]
         expect = -1 if kept.size == 0 else int(np.minimum(kept, N - kept).max())
         assert a[j] == expect
 
 
+def test_params_struct_mirror_has_the_c_layout():
+    """CParams mirrors chns_params field for field: the defaults written by the
+    library land in the fields of the same names (dealias, the last field,
+    included)."""
+    p = C.params("binary")
+    assert (p.Nx, p.model, p.sigma, p.lambda_, p.rho1, p.g, p.vpm, p.sponge, p.dealias) == \
+        (1024, 1, 2.0, 1e-6, 1.044, 3.0, 0, 0, 0)
+    assert C.params("ternary", dealias=2).dealias == 2
+
+
 def test_invalid_params_rejected():
     lib = C.load_library()
+    for bad in (dict(dealias=3), dict(dealias=1, smooth_filter=1)):
+        p = C.params("ternary", Nx=64, Ny=64, **bad)
+        assert lib.chns_dealias_extent(C.ctypes.byref(p), np.zeros(33, np.int32).ctypes.data_as(
+            C.ctypes.POINTER(C.ctypes.c_int))) == C.CHNS_E_ARG
     p = C.params("ternary", Nx=100, Ny=100)
     out = np.zeros(100 * 51)
     assert lib.chns_etd_tables(C.ctypes.byref(p), *[C._dp(out)] * 6) == C.CHNS_E_ARG

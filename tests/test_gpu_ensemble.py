@@ -140,3 +140,44 @@ def test_invalid_member_rejected():
     ps[1].Nx = 48   # not a power of two
     with pytest.raises(C.ChnsError):
         C.Ensemble(ps)
+
+
+@pytest.mark.parametrize("tuned", [False, True])
+def test_c4_workload(tuned):
+    """The batched-ensemble workload of the benchmark (bench.ensemble_members():
+    32 x 256^2 + 16 x 512^2 + 4 x 1024^2, binary and ternary alternating, each
+    with its own seed and sigma): 3 direct steps, then 9 (one 8-step graph
+    replay and one direct step; after tune() with its chosen launch splits),
+    every member equal to a stand-alone context at 1e-13, and one member of
+    each size against the oracle after 3 steps at 1e-10."""
+    import bench
+
+    members = bench.ensemble_members()
+    assert len(members) == 52
+    ps = [C.params(m, **kw) for m, kw, _ in members]
+    print(f"C4 ensemble: {len(members)} members, tuned={tuned}", flush=True)
+    with C.Ensemble(ps) as ens:
+        assert sorted(ens.groups()) == [(8, 1, 16), (8, 2, 16), (9, 1, 8), (9, 2, 8), (10, 1, 2), (10, 2, 2)]
+        for b, (m, kw, f) in enumerate(members):
+            ens[b].set_real_state(None, *f)
+        if tuned:
+            splits = ens.tune(reps=1)
+            assert all(s in (1, 2, 4) for s in splits)
+        ens.step(3)
+        early = {b: ens[b].get_state() for b in (1, 32, 49)}   # ternary members of 256^2, 512^2, 1024^2
+        ens.step(9)
+        ens.sync()
+        got = [ens[b].get_state() for b in range(len(members))]
+        status = [ens[b].status() for b in range(len(members))]
+    assert status == [0] * len(members)
+    for b, (p, (m, kw, f)) in enumerate(zip(ps, members)):
+        with C.Context(p) as ctx:
+            ctx.set_real_state(None, *f)
+            ctx.step(12)
+            ref = ctx.get_state()
+        assert close(got[b], ref, TOL_SAME), b
+    for b, g in early.items():
+        m, kw, f = members[b]
+        op = O.Params(model=m, **kw)
+        s3 = O.run(op, O.state_from_real(op, None, *f), 3)
+        assert close(g, s3.fields(), 1e-10), b

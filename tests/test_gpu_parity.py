@@ -293,3 +293,31 @@ def test_state_round_trip_is_exact(model, N):
         assert np.array_equal(a, b)
     for a, b in zip((g[::-1] if nf > 1 else g), h):
         assert np.array_equal(a, b)
+
+
+def test_ternary_linear_growth_rate():
+    """One library step of a small mode on c1, then on c2, of a uniform ternary
+    state at rest maps the pair of amplitudes by exp(A dt), with A from the
+    linearised ternary chemical potential (tests/test_oracle_pins.py derives A
+    with sympy and pins the oracle the same way): this checks
+    L_c = -0.75 lambda eps k^4, the beta/z1c/z2c tables on the device and the
+    -(lambda/gamma_i) k^2 M_i term independently of the oracle."""
+    import test_oracle_pins as T
+
+    p, c10, c20, kx, ky = T.ternary_growth_case()
+    expA, A = T.ternary_linear_map(p, c10, c20, kx, ky)
+    kw = dict(Nx=p.Nx, Ny=p.Ny, lx=p.lx, ly=p.ly, nu=p.nu, dt=p.dt, lam=p.lam, epsilon=p.epsilon,
+              sigma_12=p.sigma_12, sigma_13=p.sigma_13, sigma_23=p.sigma_23)
+    G = np.zeros((2, 2))
+    with C.Context(C.params("ternary", **kw)) as ctx:
+        for j, s0 in enumerate(T.ternary_growth_states(p, c10, c20, kx, ky)):
+            ctx.set_state(*s0.fields())
+            ctx.step(1)
+            g = ctx.get_state()
+            d0 = (s0.cphi1, s0.cphi2)[j][kx, ky]
+            for i in range(2):
+                r = g[1 + i][kx, ky] / d0
+                assert abs(r.imag) < 1e-14
+                G[i, j] = r.real
+            assert np.max(np.abs(g[0])) < 1e-12 * abs(d0)
+    T.check_ternary_growth(G, expA, A, p.dt)

@@ -142,3 +142,31 @@ def test_filter_rejections():
         ctx.set_real_state(None, np.full((32, 32), 0.3), np.full((32, 32), 0.3))
         with pytest.raises(C.ChnsError):
             ctx.adj_forward(1, 1)                                  # the adjoint covers the plain step
+
+
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+@pytest.mark.parametrize("N", [32, 64])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_step_after_state_truncation(model, N, graphs):
+    """k steps, spectrum()'s truncation of the phase fields, k steps, ... as the
+    driver runs it every navg/2 steps (main.c:281-284).  The filter is not
+    idempotent, so the steps after a truncation must read the truncated state
+    (spectrum.c:24-26, then chem_potential.c:27-32): compared with the oracle's
+    loop with the truncation in it."""
+    op, cp, reals = case(model, N)
+    s0 = full_state(op, reals)
+    k = 5
+    ref = O.run(op, s0, 4 * k, spectrum_every=k)
+    with C.Context(cp) as ctx:
+        ctx.graphs(graphs)
+        ctx.set_state(*s0.fields())
+        for _ in range(4):
+            ctx.step(k)
+            ctx.truncate_state()
+        g = ctx.get_state()
+    for a, b in zip(g, ref.fields()):
+        assert O.rel_l2(a, b) < 1e-10
+    # and the filtered trajectory does differ from one without the truncations
+    plain = O.run(op, s0, 4 * k)
+    assert max(O.rel_l2(a, b) for a, b in zip(g[1:], plain.fields()[1:])) > 1e-8
+

@@ -44,6 +44,7 @@ def same(g, d):
 @pytest.mark.parametrize("model,N,over", [
     ("ternary", 64, {}), ("binary", 128, {}), ("ns", 64, {}),
     ("ternary", 32, dict(gravity=1, vpm=1, sponge=1)), ("ternary", 32, dict(smooth_filter=1)),
+    ("binary", 64, dict(dealias=1)),
 ])
 def test_graph_replays_match_direct_launches(model, N, over):
     op, cp, reals = case(model, N, **over)
@@ -89,3 +90,26 @@ def test_graph_matches_oracle_after_reupload():
         g = ctx.get_state()
     for a, b in zip(g, ref.fields()):
         assert O.rel_l2(a, b) < 1e-10
+
+
+@pytest.mark.parametrize("model", ["ternary", "ns"])
+def test_short_first_call_leaves_the_graph_ready(model):
+    """The graph is captured on the first call whatever its length: a 5-step
+    first call (a warm-up) launches directly, and the next call replays.  The
+    results equal a context with graphs switched off, bit for bit."""
+    op, cp, reals = case(model, 64)
+    s0 = O.State(*[O.dfft(r) for r in reals])
+    with C.Context(cp) as g, C.Context(cp) as d:
+        d.graphs(False)
+        for ctx in (g, d):
+            ctx.set_state(*s0.fields())
+            ctx.step(5)
+        same(g, d)
+        for ctx in (g, d):
+            ctx.step(20)
+        same(g, d)
+        g.graphs(False)                      # switching off between calls
+        d.graphs(True)
+        for ctx in (g, d):
+            ctx.step(9)
+        same(g, d)

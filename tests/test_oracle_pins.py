@@ -68,6 +68,29 @@ def test_truncation_keep_count_is_lattice_count(N, K):
     assert int(O.dealias_mask(p).sum()) == K
 
 
+@pytest.mark.parametrize("N", [16, 64, 2048])
+def test_two_thirds_rule_keep_count_is_lattice_count(N):
+    """dealias=1: modes with |kx index| <= N/3 and ky index <= N/3 are kept;
+    dealias=2: the circle 9 (i^2 + j^2) <= N^2 (integer arithmetic here)."""
+    p1 = O.Params(Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI, dealias=1)
+    p2 = O.Params(Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI, dealias=2)
+    third = N // 3                      # N is not a multiple of 3: floor(N/3) < N/3
+    square = (2 * third + 1) * (third + 1)
+    circle = 0
+    for i in range(-N // 2 + 1, N // 2 + 1):
+        for j in range(0, N // 2 + 1):
+            circle += 9 * (i * i + j * j) <= N * N
+    m1, m2 = O.dealias_mask(p1), O.dealias_mask(p2)
+    assert int(m1.sum()) == square
+    assert int(m2.sum()) == circle
+    # the reference rule keeps a subset of both
+    m0 = O.dealias_mask(O.Params(Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI))
+    assert not np.any(m0 & ~m1) and not np.any(m0 & ~m2)
+    # kx index N/3 + 1 is cut on the ky = 0 row, N/3 kept (x index i, its negative N - i)
+    assert m1[third, 0] and m1[N - third, 0] and not m1[third + 1, 0] and not m1[N - third - 1, 0]
+    assert m1[0, third] and not m1[0, third + 1]
+
+
 # ---------------------------------------------------------------- velocity / curl
 def test_curl_of_velocity_is_vorticity_and_divergence_free():
     p = O.Params(Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
@@ -252,6 +275,133 @@ def test_etd_coefficients_match_closed_form_where_well_conditioned():
     phi2 = (ex - c[good] * p.dt) / (p.dt * c[good] ** 2)
     assert np.allclose(tb["z2w"][good], phi2, rtol=1e-9, atol=0)
     assert tb["z1w"][0, 0] == p.dt and tb["z2w"][0, 0] == 0.5 * p.dt
+
+
+def test_ternary_etd_coefficients_match_closed_form():
+    """Ternary phase-field coefficients: the linear operator of
+    dc_i/dt = -(lambda/gamma_i) k^2 (M_i + 0.75 gamma_i eps k^2 c_i) is
+    L_c = -0.75 lambda eps k^4 (the gamma_i cancel), so beta = exp(L_c dt),
+    z1c = expm1(L_c dt)/L_c and z2c = (expm1(L_c dt) - L_c dt)/(dt L_c^2) wherever
+    |L_c dt| is large enough for the reference expressions to keep their digits;
+    a wrong power of k or a missing factor changes them at O(1)."""
+    p = O.Params(model="ternary", Nx=64, Ny=64, lx=TWO_PI, ly=TWO_PI, nu=0.3, dt=0.05, lam=0.5, epsilon=0.7)
+    tb = O.etd_tables(p)
+    kx = np.fft.fftfreq(64, 1.0 / 64)[:, None]
+    ky = np.arange(33)[None, :]
+    k4 = (kx * kx + ky * ky) ** 2
+    Lc = -0.75 * p.lam * p.epsilon * k4
+    good = np.abs(Lc * p.dt) > 1e-2
+    assert good.sum() > 1000
+    ex = np.expm1(Lc[good] * p.dt)
+    assert np.allclose(tb["beta"][good], np.exp(Lc[good] * p.dt), rtol=1e-14, atol=0)
+    assert np.allclose(tb["z1c"][good], ex / Lc[good], rtol=1e-13, atol=0)
+    assert np.allclose(tb["z2c"][good], (ex - Lc[good] * p.dt) / (p.dt * Lc[good] ** 2), rtol=1e-9, atol=0)
+    assert tb["beta"][0, 0] == 1.0 and tb["z1c"][0, 0] == p.dt and tb["z2c"][0, 0] == 0.5 * p.dt
+
+
+def ternary_linear_map(p, c10, c20, kx, ky):
+    """Exact one-step map exp(A dt) of a small mode d_i cos(kx x + ky y) on a
+    uniform ternary state at rest, from the linearised chemical potential.
+
+    dc_i/dt = -(lambda/gamma_i) k^2 mu_i, mu_i = muF_i(c) + 0.75 gamma_i eps k^2 c_i
+    (chem_potential.c:74-86, surface_tension.c:116-124, nonlin.c:369-370), so
+    A = -0.75 lambda eps k^4 I - lambda k^2 diag(1/gamma_i) J with J the Jacobian
+    of (muF_1, muF_2) at (c10, c20), derived here with sympy from the
+    chemical potential's definition."""
+    import scipy.linalg
+
+    g1, g2, g3, dl = p.gammas
+    a, b = sp.symbols("a b")
+    c3 = 1 - a - b
+    fc = lambda c: c * (1 - c) * (1 - 2 * c)
+    X6 = 6 * sp.nsimplify(p.Xlambda)
+    G1 = g1 * fc(a) + X6 * a * b * c3 * b * c3
+    G2 = g2 * fc(b) + X6 * a * b * c3 * a * c3
+    G3 = g3 * fc(c3) + X6 * a * b * c3 * a * b
+    fac = 2 * dl / p.epsilon
+    mu1 = ((G1 - G2) / g2 + (G1 - G3) / g3) * fac
+    mu2 = ((G2 - G1) / g1 + (G2 - G3) / g3) * fac
+    J = np.array([[float(sp.diff(m, v).subs({a: c10, b: c20})) for v in (a, b)] for m in (mu1, mu2)])
+    k2 = float(kx * kx + ky * ky)
+    A = -0.75 * p.lam * p.epsilon * k2 * k2 * np.eye(2) - p.lam * k2 * np.diag([1.0 / g1, 1.0 / g2]) @ J
+    return scipy.linalg.expm(A * p.dt), A
+
+
+def ternary_growth_case():
+    N = 32
+    p = O.Params(model="ternary", Nx=N, Ny=N, lx=TWO_PI, ly=TWO_PI, nu=5e-3, dt=1e-3, lam=1e-3,
+                 epsilon=1.5 * TWO_PI / N, sigma_12=0.1, sigma_13=0.08, sigma_23=0.06)
+    return p, 0.3, 0.25, 3, 2
+
+
+def ternary_growth_states(p, c10, c20, kx, ky, d=1e-9):
+    """Two initial states: the mode on c1 only, then on c2 only."""
+    x, y = grid(p)
+    m = d * np.cos(kx * x + ky * y)
+    w = np.zeros((p.Nx, p.Ny2), np.complex128)
+    return [O.State(w, O.dfft(c10 + m), O.dfft(c20 + 0 * m)), O.State(w, O.dfft(c10 + 0 * m), O.dfft(c20 + m))]
+
+
+def check_ternary_growth(G, expA, A, dt):
+    """One ETD2RK step maps the mode by G (2 x 2); it must equal exp(A dt) to the
+    scheme's O((|A| dt)^3) local error.  Compared as G - I against expm(A dt) - I
+    (|A| dt ~ 1e-3, where the scheme's error is ~2e-7 of |expm(A dt) - I|, under
+    the bar of 1e-6): a dropped z2 correction misses an O(|A| dt / 2) part of
+    it, a wrong power of k in L_c or a wrong lambda/gamma_i an O(1) part."""
+    D = expA - np.eye(2)
+    assert np.max(np.abs(A)) * dt > 1e-4
+    assert np.max(np.abs((G - np.eye(2)) - D)) < 1e-6 * np.max(np.abs(D))
+
+
+def test_ternary_linear_growth_rate():
+    p, c10, c20, kx, ky = ternary_growth_case()
+    expA, A = ternary_linear_map(p, c10, c20, kx, ky)
+    # unequal gammas: the two fields grow differently and couple
+    assert abs(A[0, 1]) > 0.1 * abs(A[0, 0]) and abs(A[0, 0] - A[1, 1]) > 0.1 * abs(A[0, 0])
+    G = np.zeros((2, 2))
+    for j, s0 in enumerate(ternary_growth_states(p, c10, c20, kx, ky)):
+        s1 = O.etd2rk(p, s0)
+        d0 = (s0.cphi1, s0.cphi2)[j][kx, ky]
+        for i, f in enumerate((s1.cphi1, s1.cphi2)):
+            r = f[kx, ky] / d0
+            assert abs(r.imag) < 1e-14
+            G[i, j] = r.real
+        # a single mode drives no flow at first order
+        assert np.max(np.abs(s1.cwz)) < 1e-12 * abs(d0)
+    check_ternary_growth(G, expA, A, p.dt)
+
+
+def test_ternary_growth_pin_catches_planted_errors():
+    """The growth-rate check fails for the mistakes it is meant to catch,
+    planted in copies of the oracle's coefficient tables and parameters."""
+    p, c10, c20, kx, ky = ternary_growth_case()
+    expA, A = ternary_linear_map(p, c10, c20, kx, ky)
+    states = ternary_growth_states(p, c10, c20, kx, ky)
+
+    def response(tables=None, q=None):
+        q = q or p
+        G = np.zeros((2, 2))
+        for j, s0 in enumerate(states):
+            s1 = O.etd2rk(q, s0, tables)
+            d0 = (s0.cphi1, s0.cphi2)[j][kx, ky]
+            G[:, j] = [(s1.cphi1[kx, ky] / d0).real, (s1.cphi2[kx, ky] / d0).real]
+        return G
+
+    tb = O.etd_tables(p)
+    ks = O.ksqr_grid(p)
+    bad = []
+    t = dict(tb)
+    t["z2c"] = np.zeros_like(tb["z2c"])                       # dropped z2 term
+    bad.append(response(t))
+    t = dict(tb)
+    Lc = -0.75 * p.lam * p.epsilon * ks ** 3                  # k^6 instead of k^4
+    t["beta"] = np.exp(Lc * p.dt)
+    t["z1c"] = np.where(Lc != 0, np.expm1(Lc * p.dt) / np.where(Lc != 0, Lc, 1), p.dt)
+    bad.append(response(t))
+    bad.append(response(q=O.Params(**{**p.__dict__, "sigma_13": 0.06, "sigma_23": 0.08})))   # gammas swapped
+    for G in bad:
+        with pytest.raises(AssertionError):
+            check_ternary_growth(G, expA, A, p.dt)
 
 
 def test_stokes_decay_single_mode():
