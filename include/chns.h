@@ -79,8 +79,9 @@ const char *chns_last_error(const chns_ctx *ctx);
 
 /* Upload / download the spectral state (host arrays, IDXC layout, 2*Nx*Ny2
  * doubles each).  cphi1/cphi2 are ignored (may be NULL) when the model has no
- * such field; for BINARY cphi1 is the reference's cphi.  Replaces the state
- * arrays cwz, cphi1, cphi2 of global.c:13-49. */
+ * such field; for BINARY cphi1 is the reference's cphi.  Once a state is
+ * set, a NULL array in chns_set_state leaves that field as it is on the
+ * device.  Replaces the state arrays cwz, cphi1, cphi2 of global.c:13-49. */
 int chns_set_state(chns_ctx *ctx, const double *cwz, const double *cphi1, const double *cphi2);
 int chns_get_state(chns_ctx *ctx, double *cwz, double *cphi1, double *cphi2);
 

@@ -18,6 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "chns_amd", "libchns_amd.so")
+REF_OUT = os.path.join(HERE, "chns_amd", "libchns_ref.so")
 DRIVER = os.path.join(HERE, "chns_amd", "chns2d")
 INC = os.path.join(ROOT, "include")
 
@@ -72,9 +73,13 @@ def build(verbose: bool = False) -> str:
         list(ex.map(_run, steps))
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["batch"],
           objs["adjdev"], objs["filter"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
-    # time-loop driver (plain C on the C ABI), found next to the library via rpath
+    # the reference-named entry points (chns_ref.h) over the C ABI, and the
+    # time-loop driver on them (plain C), found next to the library via rpath
+    _run(["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-Wextra", "-fPIC", "-shared", "-I" + INC,
+          os.path.join(HERE, "host", "chns_ref.c"), "-o", REF_OUT, "-L" + os.path.dirname(OUT), "-lchns_amd",
+          "-Wl,-rpath,$ORIGIN", "-lm"])
     _run(["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-I" + INC, os.path.join(HERE, "host", "chns2d.c"),
-          "-o", DRIVER, "-L" + os.path.dirname(OUT), "-lchns_amd", "-Wl,-rpath,$ORIGIN", "-lm"])
+          "-o", DRIVER, "-L" + os.path.dirname(OUT), "-lchns_ref", "-lchns_amd", "-Wl,-rpath,$ORIGIN", "-lm"])
     if verbose:
         print("built", OUT)
     return OUT

@@ -273,7 +273,8 @@ int chns_stage_buffer(chns_ctx* c) {
 // layout on the device.
 extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi1, const double* cphi2) {
   if (!c) return CHNS_E_ARG;
-  if (!cwz || (c->nph >= 1 && !cphi1) || (c->nph >= 2 && !cphi2)) return fail(c, CHNS_E_ARG, "null state array");
+  if (!c->have_state && (!cwz || (c->nph >= 1 && !cphi1) || (c->nph >= 2 && !cphi2)))
+    return fail(c, CHNS_E_ARG, "null state array (the first upload needs every field)");
   CHNS_HIP(c, hipSetDevice(c->dev));
   const int rc = chns_stage_buffer(c);
   if (rc != CHNS_OK) return rc;
@@ -282,6 +283,7 @@ extern "C" int chns_set_state(chns_ctx* c, const double* cwz, const double* cphi
   const double* src[3] = {cwz, cphi1, cphi2};
   chns::cd* dst[3] = {c->b.W, c->b.C1, c->b.C2};
   for (int f = 0; f <= c->nph; ++f) {
+    if (!src[f]) continue;   // keeps the field on the device
     CHNS_HIP(c, hipMemcpyAsync(c->dstage, src[f], bytes, hipMemcpyHostToDevice, c->stream));
     CHNS_LAUNCH(c, chns::launch_transpose(c->dstage, dst[f], N, H, c->stream));
   }

@@ -1,5 +1,7 @@
 /*
- * chns2d - time-loop driver of the MI355X CHNS-2D step (C, on the C ABI).
+ * chns2d - time-loop driver of the MI355X CHNS-2D step: main.c written over
+ * the reference-named entry points of chns_ref.h (etd2rk, free_energy,
+ * kinetic_energy, perimeter, spectrum), which call the C ABI of chns.h.
  *
  * Mirrors the reference main.c:24-368 on one GPU: initial condition
  * (init.c), t = 0 diagnostics and dump, then for it = 1..maxiter:
@@ -26,6 +28,8 @@
  *   read1=path read2=path (raw float64 [Nx][Ny], INIT_READ_DATA; defaults
  *   init_lens/phi1.in, init_lens/phi2.in, binary init_field/phi.in)
  *   diag_every=K  out=DIR  device=D
+ *   mirror=1: the host arrays hold the state across every call (the
+ *   reference's in-place semantics; see chns_ref.h), else device-resident
  *   ic_only=1: write the initial real fields to init_field/ic{1,2}.bin and
  *   stop before any device call
  * Defaults are the reference defaults (TERNARY, N = 1024, L = 4 pi, ...).
@@ -38,16 +42,7 @@
 #include <unistd.h>
 
 #include "chns.h"
-
-static void die(const char *what, chns_ctx *c) {
-  fprintf(stderr, "chns2d: %s: %s\n", what, c ? chns_last_error(c) : "");
-  exit(1);
-}
-
-#define CK(call, ctx) \
-  do {                \
-    if ((call) != CHNS_OK) die(#call, ctx); \
-  } while (0)
+#include "chns_ref.h"
 
 static int read_raw(const char *path, double *u, size_t n) {
   FILE *f = fopen(path, "rb");
@@ -77,7 +72,7 @@ int main(int argc, char **argv) {
   double maxiter = 2e5, nfile = 1000;          /* global.h:32-33 */
   const char *init = model == CHNS_MODEL_BINARY ? "droplet" : (model == CHNS_MODEL_TERNARY ? "noise" : "zero");
   const char *read1 = model == CHNS_MODEL_BINARY ? "init_field/phi.in" : "init_lens/phi1.in", *read2 = "init_lens/phi2.in", *out = ".";
-  int nranks = 1, diag_every = 1, device = 0, ic_only = 0;
+  int nranks = 1, diag_every = 1, device = 0, ic_only = 0, mirror = 0;
   for (int a = 1; a < argc; ++a) {
     char *eq = strchr(argv[a], '=');
     if (!eq) continue;
@@ -115,6 +110,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(k, "out")) out = v;
     else if (!strcmp(k, "device")) device = atoi(v);
     else if (!strcmp(k, "ic_only")) ic_only = atoi(v);
+    else if (!strcmp(k, "mirror")) mirror = atoi(v);
     else if (strcmp(k, "model")) {
       fprintf(stderr, "chns2d: unknown parameter %s\n", k);
       return 2;
@@ -142,7 +138,6 @@ int main(int argc, char **argv) {
   const size_t npt = (size_t)Nx * Ny;
   double *f1 = nph ? malloc(npt * sizeof(double)) : NULL;
   double *f2 = nph == 2 ? malloc(npt * sizeof(double)) : NULL;
-  double *work = malloc(npt * sizeof(double));
   if (!strcmp(init, "read")) {                 /* init.c:220-225, 252-262 */
     if (nph >= 1 && read_raw(read1, f1, npt)) { fprintf(stderr, "chns2d: cannot read %s\n", read1); return 1; }
     if (nph == 2 && read_raw(read2, f2, npt)) { fprintf(stderr, "chns2d: cannot read %s\n", read2); return 1; }
@@ -175,105 +170,58 @@ int main(int argc, char **argv) {
     if (nph == 2) write_raw("init_field/ic2.bin", f2, npt);
     return 0;
   }
-  chns_ctx *ctx = NULL;
-  if (chns_create(&ctx, &p, device) != CHNS_OK) die("chns_create", NULL);
-  CK(chns_set_real_state(ctx, NULL, f1, f2), ctx);   /* zero vorticity (ZERO_INIT, init.c:89-100) + truncation */
+  /* ---- main.c:37-194 and init.c:291-294 through the reference-named entry
+   * points of chns_ref.h (the state lives on the device; with mirror=1 the
+   * host arrays cwz, cphi1, cphi2 are the state, as in the reference) */
+  if (chns_ref_init(&p, device, mirror) != CHNS_OK) {
+    fprintf(stderr, "chns2d: chns_ref_init failed (invalid parameters or no device)\n");
+    return 1;
+  }
+  const size_t nc = (size_t)Nx * (Ny / 2 + 1);
+  double (*cwz)[2] = malloc(nc * sizeof *cwz);
+  double (*cphi1)[2] = nph >= 1 ? malloc(nc * sizeof *cphi1) : NULL;
+  double (*cphi2)[2] = nph == 2 ? malloc(nc * sizeof *cphi2) : NULL;
+  chns_ref_init_real(NULL, f1, f2, cwz, cphi1, cphi2);   /* zero vorticity (ZERO_INIT, init.c:89-100) */
 
   FILE *fu = fopen("data/ken_t.dat", "w");
   FILE *fe = nph ? fopen("data/free_en_t.dat", "w") : NULL;
   FILE *fdef1 = nph ? fopen(nph == 2 ? "data/gamma1.dat" : "data/gamma.dat", "w") : NULL;
   FILE *fdef2 = nph == 2 ? fopen("data/gamma2.dat", "w") : NULL;
   FILE *fph = nph ? fopen(nph == 2 ? "data/volume_phase1.dat" : "data/volume_droplet.dat", "w") : NULL;
-  int nshell = 0, nxy = 0;
-  chns_diag_sizes(&p, &nshell, &nxy);
-  double *sp[7];
-  for (int q = 0; q < 7; ++q) sp[q] = malloc((size_t)(q < 5 ? nshell : nxy) * sizeof(double));
-  const double factor = (2 * M_PI) / p.lx;
-  const int kxm = (int)lround(sqrt(((Nx / 2.0) * factor) * ((Nx / 2.0) * factor)));   /* main.c:62 */
+  chns_ref_outputs(fu, fe, fdef1, fdef2);
   double t = 0.0;
-  int ifile = 0, fl = 0;
 
-  #define DIAGNOSTICS()                                                                             \
-    do {                                                                                            \
-      double eu = 0, e1 = 0, e2 = 0;                                                                \
-      if (nph) {                                                                                    \
-        double ephi, egphi;                                                                         \
-        CK(chns_free_energy(ctx, &ephi, &egphi), ctx);                                              \
-        fprintf(fe, "%lf \t %.12lf \t %.12lf\n", t, ephi, egphi);                                   \
-        if (chns_status(ctx, NULL) == CHNS_E_DIVERGED) {  /* free_energy.c:151-154 */              \
-          fprintf(stderr, "chns2d: %s\n", chns_last_error(ctx));                                    \
-          exit(3);                                                                                  \
-        }                                                                                           \
-      }                                                                                             \
-      CK(chns_kinetic_energy(ctx, &eu, &e1, &e2), ctx);                                             \
-      if (nph == 2) fprintf(fu, "%lf \t %.12lf \t %.12lf \t %.12lf\n", t, eu, e1, e2);              \
-      else if (nph == 1) fprintf(fu, "%lf \t %.12lf \t %.12lf\n", t, eu, e1);                      \
-      else fprintf(fu, "%lf \t %.12lf\n", t, eu);                                                    \
-      if (nph) {                                                                                    \
-        double cnt[4];                                                                              \
-        CK(chns_perimeter(ctx, cnt), ctx);                                                          \
-        for (int ph = 0; ph < nph; ++ph) {                                                          \
-          const double Spt = cnt[2 * ph] * dx, Sp0 = 2.0 * sqrt(M_PI * cnt[2 * ph + 1] * dx * dy);  \
-          fprintf(ph == 0 ? fdef1 : fdef2, "%g \t %g \t %g \t %g\n", t, Sp0, Spt, (Spt / Sp0) - 1.0); \
-        }                                                                                           \
-      }                                                                                             \
-    } while (0)
-
-  #define SPECTRUM()                                                                                \
-    do {                                                                                            \
-      char fn[128];                                                                                 \
-      CK(chns_spectrum(ctx, 1, sp[0], sp[1], sp[2], sp[3], sp[4], sp[5], sp[6]), ctx);             \
-      snprintf(fn, sizeof fn, "spectra/spectrum.%d", ifile);                                        \
-      FILE *fk = fopen(fn, "w");                                                                    \
-      for (int ik = 1; ik < nshell; ++ik) {                                                         \
-        const double kn = ik * (2 * M_PI / p.lx);                                                   \
-        if (nph == 2) fprintf(fk, "%.5e \t %.12e \t %.12e \t %.12e \t %.12e \t %.12e\n", kn, sp[0][ik], sp[1][ik], sp[2][ik], sp[3][ik], sp[4][ik]); \
-        else if (nph == 1) fprintf(fk, "%.5e \t %.12e \t %.12e \t %.12e\n", kn, sp[0][ik], sp[1][ik], sp[2][ik]); \
-        else fprintf(fk, "%.5e \t %.12e \t %.12e\n", kn, sp[0][ik], sp[1][ik]);                     \
-      }                                                                                             \
-      fclose(fk);                                                                                   \
-      snprintf(fn, sizeof fn, "spectra/Xspectrum.%d", ifile);                                       \
-      fk = fopen(fn, "w");                                                                          \
-      for (int b = 1; b < kxm && b < nxy; ++b) fprintf(fk, "%d \t %.12e\n", b, sp[5][b]);           \
-      fclose(fk);                                                                                   \
-      snprintf(fn, sizeof fn, "spectra/Yspectrum.%d", ifile);                                       \
-      fk = fopen(fn, "w");                                                                          \
-      for (int b = 1; b < kxm && b < nxy; ++b) fprintf(fk, "%d \t %.12e\n", b, sp[6][b]);           \
-      fclose(fk);                                                                                   \
-      ++ifile;                                                                                      \
-    } while (0)
-
-  #define DUMP(trunc)                                                                               \
-    do {                                                                                            \
-      char fn[128];                                                                                 \
-      CK(chns_real_field(ctx, 0, trunc, work), ctx);                                                \
-      snprintf(fn, sizeof fn, "vort/wz.%d", fl);                                                    \
-      write_raw(fn, work, npt);                                                                     \
-      for (int ph = 1; ph <= nph; ++ph) {                                                           \
-        CK(chns_real_field(ctx, ph, trunc, work), ctx);                                             \
-        if (nph == 2) snprintf(fn, sizeof fn, "phase_field/phi%d.%d", ph, fl);                      \
-        else snprintf(fn, sizeof fn, "phase_field/phi.%d", fl);                                     \
-        write_raw(fn, work, npt);                                                                   \
-      }                                                                                             \
-      ++fl;                                                                                         \
+  /* main.c:276-279 (every diag_every steps here) */
+  #define DIAGNOSTICS()                          \
+    do {                                         \
+      chns_ref_set_time(t);                      \
+      if (nph) free_energy(cphi1, cphi2);        \
+      kinetic_energy(cwz, cphi1, cphi2);         \
+      if (nph) perimeter(cphi1, cphi2);          \
     } while (0)
 
   DIAGNOSTICS();                               /* main.c:214-228 */
-  SPECTRUM();
-  DUMP(0);                                     /* main.c:230-250 (no truncation of the copy) */
+  spectrum(cwz, cphi1, cphi2);
+  chns_ref_dump(0);                            /* main.c:230-250 (no truncation of the copy) */
   printf("\nTime stepping start.............\n");
   for (long it = 1; it <= (long)maxiter; ++it) {   /* main.c:260-367 */
     t += p.dt;
-    CK(chns_step(ctx, 1), ctx);
+    chns_ref_set_time(t);
+    etd2rk(cwz, cphi1, cphi2);
     if (it % diag_every == 0) DIAGNOSTICS();
-    if (it % (navg / 2) == 0) SPECTRUM();
-    if (it % navg == 0) DUMP(1);
+    if (it % (navg / 2) == 0) spectrum(cwz, cphi1, cphi2);
+    if (it % navg == 0) chns_ref_dump(1);
   }
   fclose(fu);
   if (fe) fclose(fe);
   if (fdef1) fclose(fdef1);
   if (fdef2) fclose(fdef2);
   if (fph) fclose(fph);
-  chns_destroy(ctx);
+  chns_ref_finalize();
+  free(cwz);
+  free(cphi1);
+  free(cphi2);
+  free(f1);
+  free(f2);
   return 0;
 }

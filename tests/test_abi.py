@@ -25,6 +25,20 @@ def test_library_exports_every_declared_symbol():
     assert set(declared_symbols()) == set(C.EXPORTS)
 
 
+def test_reference_named_library_exports_its_header():
+    """libchns_ref.so exports every function include/chns_ref.h declares: the
+    reference's etd2rk, free_energy, kinetic_energy, perimeter, spectrum with
+    their names, and the chns_ref_* set-up calls."""
+    import ctypes
+    txt = open(os.path.join(ROOT, "include", "chns_ref.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|void)\s+([a-z0-9_]+)\s*\(", txt, re.M)))
+    assert {"etd2rk", "free_energy", "kinetic_energy", "perimeter", "spectrum"} <= set(names)
+    C.load_library()
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(C.LIB_PATH), "libchns_ref.so"))
+    for name in names:
+        assert hasattr(lib, name), name
+
+
 CASES = [
     ("ns", 64, 2 * math.pi), ("binary", 128, 2 * math.pi), ("ternary", 128, 4 * math.pi),
     ("ternary", 2048, 2 * math.pi), ("binary", 512, 4 * math.pi),

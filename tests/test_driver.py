@@ -111,6 +111,10 @@ CASES = {
                       dict(Nx=32, Ny=32, model="binary", smooth_filter=True), "droplet"),
     # the 2/3 rule instead of the reference's N/4 circle
     "ternary_23": (dict(Nx=32, dealias=1), dict(Nx=32, Ny=32, dealias=1), "noise"),
+    # the host arrays hold the state across every reference-named call
+    # (chns_ref.h mirror mode: upload, run, write back, as the reference does)
+    "ternary_mirror": (dict(Nx=32, mirror=1), dict(Nx=32, Ny=32), "noise"),
+    "binary_mirror": (dict(model="binary", Nx=32, mirror=1), dict(Nx=32, Ny=32, model="binary"), "droplet"),
 }
 
 
