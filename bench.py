@@ -10,8 +10,9 @@ five-pass design, coefficient tables excluded) over its mean launch duration,
 from HIP events recorded on the library's stream around every launch during K
 further steps right after the timed region (events between launches cost
 about 7% of the step, so the timed region itself carries none).
-``cpu_baseline`` times the oracle (its FFTs on all host cores through
-scipy.fft, at most 16) on a bounded sample of the same workload.  ``roofline.traffic`` is the HBM bytes per launch of that
+``cpu_baseline`` times the C/OpenMP restatement of the step (oracle/chns_step.c)
+on the host cores (at most 16, one GPU's share) on a bounded sample of the same
+workload, and records the CPU model.  ``roofline.traffic`` is the HBM bytes per launch of that
 pass from two child ``rocprofv3 --pmc`` runs of this script (FETCH_SIZE, then
 WRITE_SIZE, KiB per dispatch), each scaled by the factor that makes the same
 counter read the true size of a plain 256 MiB device-to-device copy
@@ -126,34 +127,54 @@ def pass_bytes(N, Jt, K):
     }, 42 * F + 48 * Ft + 6 * Fk
 
 
-def cpu_baseline(N, kw, seconds=10.0, max_steps=50, threads=None):
-    """The oracle stepping the same workload until ``seconds`` of CPU work have
-    elapsed, its 2-D transforms run by scipy.fft with ``threads`` workers (all
-    the host cores this process may use, at most 16) instead of numpy's
-    single-threaded pocketfft: the same algorithm on every core of the host."""
-    import scipy.fft as sf
+def host_cpu():
+    """CPU model (/proc/cpuinfo) and the core counts the OS reports."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
+
+def cpu_baseline(N, kw, warm=2, steps=20, max_seconds=30.0, threads=None):
+    """The C/OpenMP restatement of the step (oracle/chns_step.c: the same
+    reference functions as the numpy oracle, its own radix-2 FFT, two real
+    fields per complex transform) on the host cores: ``warm`` untimed steps,
+    then up to ``steps`` timed ones (fewer if ``max_seconds`` runs out), on the
+    lens workload of the GPU leg.  ``threads``: the cores this process may use,
+    at most 16 (one GPU's share of the host)."""
     import oracle as O
+    from oracle import c_oracle as CO
 
     if threads is None:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    O.dfft = lambda u: sf.rfft2(u, workers=threads)
-    O.ifft = lambda p, cu: sf.irfft2(cu, s=(p.Nx, p.Ny), workers=threads)
+    used = CO.threads(threads)
     op = O.Params(model="ternary", **kw)
-    c1, c2 = O.init_lens_real(op)
-    s = O.state_from_real(op, None, c1, c2)
-    tb = O.etd_tables(op)
-    steps = 0
-    t0 = time.perf_counter()
-    while steps < max_steps:
-        s = O.etd2rk(op, s, tb)
-        steps += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": N * N * steps / dt, "unit": "grid-pt steps/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} ternary ETD2RK steps of the {N}^2 lens workload (numpy oracle, scipy.fft with "
-                      f"{threads} workers), {dt:.1f} s"}
+    s0 = O.state_from_real(op, None, *O.init_lens_real(op))
+    st = CO.Stepper(op)
+    try:
+        st.set_state(s0)
+        st.step(warm)
+        done = 0
+        t0 = time.perf_counter()
+        while done < steps:
+            st.step(1)
+            done += 1
+            if time.perf_counter() - t0 >= max_seconds:
+                break
+        dt = time.perf_counter() - t0
+    finally:
+        st.close()
+    cpu = host_cpu()
+    return {"value": N * N * done / dt, "unit": "grid-pt steps/s", "cores": used, "kind": "port",
+            "cpu_model": cpu["model"], "nproc": cpu["nproc"], "affinity": cpu["affinity"],
+            "sample": f"{done} ternary ETD2RK steps of the {N}^2 lens workload after {warm} warm-up steps "
+                      f"(C/OpenMP restatement oracle/chns_step.c, {used} threads), {dt:.1f} s"}
 
 
 def ensemble_members():

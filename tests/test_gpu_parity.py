@@ -321,3 +321,58 @@ def test_ternary_linear_growth_rate():
                 G[i, j] = r.real
             assert np.max(np.abs(g[0])) < 1e-12 * abs(d0)
     T.check_ternary_growth(G, expA, A, p.dt)
+
+
+
+def full_size_case(model, N):
+    """Workloads at the benchmark sizes that stay finite over 100 steps:
+    NS with forcing; the binary spinodal quench and the forced ternary of the
+    benchmark configurations (dt scaled as (256/N)^3, the explicit k^2 muF
+    rate growing as N^3 at eps = 1.5 dx); at 2048^2 the liquid lens of the
+    benchmark (dt = 1e-6) with a random vorticity."""
+    import bench
+
+    L = 2 * math.pi
+    if model == "ns":
+        kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=2e-4, f_amp=0.1, f_k=4)
+        op = O.Params(model="ns", **kw)
+        return op, C.params("ns", **kw), O.state_from_real(op, O.random_vorticity(op, 1))
+    if model == "binary":
+        kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=0.01, dt=5e-4 * (256 / N) ** 3, lam=1e-3, epsilon=1.5 * L / N, sigma=0.1)
+        op = O.Params(model="binary", **kw)
+        return op, C.params("binary", **kw), O.state_from_real(op, None, *O.init_noise_real(op, 1, 0.5, 0.1))
+    if N < 2048:
+        kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=5e-4 * (256 / N) ** 3, lam=1e-3, epsilon=1.5 * L / N,
+                  sigma_12=0.1, sigma_13=0.1, sigma_23=0.1, Xlambda=2.0, f_amp=0.1, f_k=4)
+        op = O.Params(model="ternary", **kw)
+        return op, C.params("ternary", **kw), O.state_from_real(op, None, *O.init_noise_real(op, 2, 1 / 3, 0.05))
+    kw = bench.workload(N)
+    op = O.Params(model="ternary", **kw)
+    return op, C.params("ternary", **kw), O.state_from_real(op, O.random_vorticity(op, 1), *O.init_lens_real(op))
+
+
+@pytest.mark.parametrize("model,N", [("ns", 1024), ("binary", 1024), ("ternary", 1024), ("ternary", 2048)])
+def test_100_steps_full_size(model, N):
+    """100 steps at the benchmark sizes: rel-L2 <= 1e-10 of the state and 1e-8
+    of its change over the run, mass to 1e-13, against the C/OpenMP
+    restatement of the oracle (oracle/chns_step.c, pinned to the numpy oracle
+    to round-off in tests/test_c_oracle.py), the spectrum truncation every 25
+    steps."""
+    from oracle import c_oracle as CO
+
+    op, cp, s0 = full_size_case(model, N)
+    CO.threads(16)
+    print(f"100 steps {model} {N}^2 (C oracle, 16 threads)", flush=True)
+    ref = CO.run(op, s0, 100, spectrum_every=25)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        for _ in range(4):
+            ctx.step(25)
+            ctx.truncate_state()
+        g = ctx.get_state()
+        assert ctx.status() == 0
+    for a, b, a0 in zip(g, ref.fields(), s0.fields()):
+        assert rel(a, b) <= TOL_TRAJ
+        assert rel(a - a0, b - a0) <= 1e-8
+    for a, b in zip(g[1:], s0.fields()[1:]):
+        assert abs(a[0, 0] - b[0, 0]) <= TOL_MASS * abs(b[0, 0])
