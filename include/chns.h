@@ -135,6 +135,15 @@ int chns_truncate_state(chns_ctx *ctx);
 /* Bulk and gradient free energy of the untruncated phase fields
  * (free_energy.c:13-157); BINARY and TERNARY only. */
 int chns_free_energy(chns_ctx *ctx, double *ephi, double *egphi);
+/* chns_free_energy without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * or time loops, harnesses that forbid hipMalloc).  Same outputs, bit for
+ * bit, and the same free-energy guard for chns_status.  Its reduction
+ * partials live in a static device array of 41 KB per GPU (N <= 2048; larger
+ * N returns CHNS_E_ARG) and its transforms reuse the context's step buffers,
+ * which hold nothing between steps.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (two threads or streams) race. */
+int chns_free_energy_noalloc(chns_ctx *ctx, double *ephi, double *egphi);
 /* Mean kinetic energy eu and the per-phase kinetic energies averaged over
  * the points with c_i >= 0.5 (kinetic_energy.c:3-144); eph2 is ternary only,
  * eph1/eph2 untouched for NS.  Any output may be NULL. */

@@ -36,7 +36,7 @@ EXPORTS = (
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
     "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store", "chns_graphs_enable",
-    "chns_status",
+    "chns_status", "chns_free_energy_noalloc",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -124,6 +124,7 @@ def load_library(path: str = LIB_PATH):
     lib.chns_adj_set_store.argtypes = [V, I]
     lib.chns_graphs_enable.argtypes = [V, I]
     lib.chns_status.argtypes = [V, ctypes.POINTER(I)]
+    lib.chns_free_energy_noalloc.argtypes = [V, D, D]
     _lib = lib
     return lib
 
@@ -253,6 +254,13 @@ class Context:
         """(ephi, egphi) of the untruncated phase fields (free_energy.c:13-157)."""
         a, b = ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.chns_free_energy(self.h, ctypes.byref(a), ctypes.byref(b)), "chns_free_energy")
+        return a.value, b.value
+
+    def free_energy_noalloc(self):
+        """free_energy() with no device allocation (static scratch, step buffers reused)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.chns_free_energy_noalloc(self.h, ctypes.byref(a), ctypes.byref(b)),
+                    "chns_free_energy_noalloc")
         return a.value, b.value
 
     def kinetic_energy(self):

@@ -173,6 +173,8 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   }
   int rc = 0;
   auto bail = [&](int r) { int code = r; std::fprintf(stderr, "chns_create: %s\n", c->err.c_str()); chns_destroy(c); return code; };
+  if ((rc = chns_noalloc_setup(device, nullptr)) != 0)
+    return bail(hip_fail(c, (hipError_t)rc, "hipGetSymbolAddress(scratch)"));
   if (shared) {
     c->stream = shared;
     c->own_stream = false;

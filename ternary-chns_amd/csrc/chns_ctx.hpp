@@ -60,3 +60,6 @@ bool chns_graphs_default();
 int chns_real_of(chns_ctx* c, const chns::cd* S, double* out);
 // frees the adjoint storage (chns_destroy, chns_adj_release)
 void chns_adj_free(chns_ctx* c);
+// device address of chns_free_energy_noalloc's static scratch array on `dev`
+// (current device), looked up once per device (chns_noalloc.hip)
+int chns_noalloc_setup(int dev, double** out);
