@@ -236,6 +236,13 @@ int chns_adj_forward(chns_ctx *ctx, int nsteps, int ckpt_every);
  * spectrum of spectrum.c:83-103 at the current (final) state, and its seed.
  * loss and sph1 (nshell entries) may be NULL.  The state is not truncated. */
 int chns_adj_seed_spectrum(chns_ctx *ctx, const double *target, double *loss, double *sph1);
+/* Loss = total free energy ephi + egphi (free_energy.c:13-157) of the
+ * truncated final phase fields T(c1), T(c2) (truncation() as every read of
+ * the state inside the step applies it; the reference's diagnostic reads the
+ * untruncated fields, whose extra modes the step never reads back), and its
+ * seed; its direct dependence on sigma_ij through the gamma_i weights enters
+ * g_sigma.  loss, ephi, egphi may be NULL.  The state is not changed. */
+int chns_adj_seed_free_energy(chns_ctx *ctx, double *loss, double *ephi, double *egphi);
 /* Seed with caller-given spectral adjoints of the final state (IDXC layout,
  * NULL = zero); they are truncated to the kept modes. */
 int chns_adj_seed(chns_ctx *ctx, const double *wbar, const double *c1bar, const double *c2bar);

@@ -211,13 +211,58 @@ def spectrum_loss(p: O.Params, s: O.State, target):
     return loss, S, [z, fac * s2.cphi1, z.copy()]
 
 
-def sigma_gradient(p: O.Params, part):
+def free_energy_loss(p: O.Params, s: O.State):
+    """Loss = ephi + egphi of free_energy.c:13-157 evaluated on the truncated
+    phase fields T(c1), T(c2), its spectral adjoint, and its direct
+    derivatives dL/dgamma_i (the gamma_i weights of the two energies).
+
+    Returns (loss, ephi, egphi, [wbar, c1bar, c2bar], dgam[3]).  With
+    s = c1 + c2 = 1 - c3, ephi = b sum f, f = g1 c1^2 (1-c1)^2 +
+    g2 c2^2 (1-c2)^2 + g3 s^2 (1-s)^2, and egphi = q sum g1 |grad c1|^2 +
+    g2 |grad c2|^2 + g3 |grad c1 + grad c2|^2 (b = 6 scale/eps,
+    q = 0.375 eps scale); c_i bar = rfft2(df/dc_i) - i kx rfft2(dg/dc_i,x)
+    - i ky rfft2(dg/dc_i,y) on the kept modes.
+    """
+    g1, g2, g3, _ = p.gammas
+    kx, ky = O.kxky(p)
+    mask = O.dealias_mask(p)
+    scale = 1.0 / (p.Nx * p.Ny)
+    b = (6.0 / p.epsilon) * scale
+    q = 0.375 * p.epsilon * scale
+    C1 = O.truncation(p, s.cphi1)
+    C2 = O.truncation(p, s.cphi2)
+    c1, c2 = O.ifft(p, C1), O.ifft(p, C2)
+    c1x, c1y = O.ifft(p, 1j * kx[:, None] * C1), O.ifft(p, 1j * ky[None, :] * C1)
+    c2x, c2y = O.ifft(p, 1j * kx[:, None] * C2), O.ifft(p, 1j * ky[None, :] * C2)
+    sm = c1 + c2
+    f1, f2, f3 = c1 * c1 * (1 - c1) ** 2, c2 * c2 * (1 - c2) ** 2, sm * sm * (1 - sm) ** 2
+    e1, e2 = c1x * c1x + c1y * c1y, c2x * c2x + c2y * c2y
+    e3 = (c1x + c2x) ** 2 + (c1y + c2y) ** 2
+    ephi = b * float(np.sum(g1 * f1 + g2 * f2 + g3 * f3))
+    egphi = q * float(np.sum(g1 * e1 + g2 * e2 + g3 * e3))
+    dgam = [b * float(np.sum(f1)) + q * float(np.sum(e1)), b * float(np.sum(f2)) + q * float(np.sum(e2)),
+            b * float(np.sum(f3)) + q * float(np.sum(e3))]
+    d3 = g3 * 2 * sm * (1 - sm) * (1 - 2 * sm)
+    x1 = b * (g1 * 2 * c1 * (1 - c1) * (1 - 2 * c1) + d3)
+    x2 = b * (g2 * 2 * c2 * (1 - c2) * (1 - 2 * c2) + d3)
+    y1x, y1y = q * 2 * (g1 * c1x + g3 * (c1x + c2x)), q * 2 * (g1 * c1y + g3 * (c1y + c2y))
+    y2x, y2y = q * 2 * (g2 * c2x + g3 * (c1x + c2x)), q * 2 * (g2 * c2y + g3 * (c1y + c2y))
+
+    def seed(x, yx, yy):
+        z = O.dfft(x) - 1j * kx[:, None] * O.dfft(yx) - 1j * ky[None, :] * O.dfft(yy)
+        return np.where(mask, z, 0.0)
+
+    z = np.zeros_like(s.cwz)
+    return ephi + egphi, ephi, egphi, [z, seed(x1, y1x, y1y), seed(x2, y2x, y2y)], dgam
+
+
+def sigma_gradient(p: O.Params, part, dgam=(0.0, 0.0, 0.0)):
     """dL/d(sigma_12, sigma_13, sigma_23) from the coefficient partials.
 
     gamma_1 = s12 - s23 + s13, gamma_2 = s12 - s13 + s23, gamma_3 = s23 - s12 + s13,
     del = 6 g1 g2 g3 / (g1 g2 + g1 g3 + g2 g3) (main.c:166-170), fac = 2 del/eps,
     mc_i = 0.75 g_i eps, A11 = 1 + g3/g1, A12 = g3/g2, A21 = g3/g1, A22 = 1 + g3/g2,
-    lg_i = lambda/g_i.
+    lg_i = lambda/g_i.  dgam: the loss's own dL/dgamma_i (free_energy_loss).
     """
     g1, g2, g3, _ = p.gammas
     eps, lam = p.epsilon, p.lam
@@ -232,24 +277,30 @@ def sigma_gradient(p: O.Params, part):
     dg2 = (part["G2"] - part["g2i"] / (g2 * g2) + 0.75 * eps * part["mc2"]
            - g3 / (g2 * g2) * (part["A12"] + part["A22"]) - lam / (g2 * g2) * part["lg2"] + rf * dd2)
     dg3 = (part["G3"] - part["g3i"] / (g3 * g3) + (part["A11"] + part["A21"]) / g1
-           + (part["A12"] + part["A22"]) / g2 + rf * dd3)
+           + (part["A12"] + part["A22"]) / g2 + rf * dd3 + dgam[2])
+    dg1 += dgam[0]
+    dg2 += dgam[1]
     return np.array([dg1 + dg2 - dg3, dg1 - dg2 + dg3, -dg1 + dg2 + dg3])
 
 
 def loss_and_grad(p: O.Params, s0: O.State, nsteps: int, target):
-    """Forward nsteps from s0 (all states kept), the spectrum loss at the end,
-    and the reverse sweep.  Returns (loss, real-space gradients [gw, gc1, gc2]
-    with respect to the real fields that set_real_state turns into s0, dL/dsigma
-    [s12, s13, s23])."""
+    """Forward nsteps from s0 (all states kept), the spectrum loss at the end
+    (target None: the free-energy loss), and the reverse sweep.  Returns (loss,
+    real-space gradients [gw, gc1, gc2] with respect to the real fields that
+    set_real_state turns into s0, dL/dsigma [s12, s13, s23])."""
     tb = O.etd_tables(p)
     traj = [s0]
     for _ in range(nsteps):
         traj.append(O.etd2rk(p, traj[-1], tb))
-    loss, _, ubar = spectrum_loss(p, traj[-1], target)
+    dgam = (0.0, 0.0, 0.0)
+    if target is None:
+        loss, _, _, ubar, dgam = free_energy_loss(p, traj[-1])
+    else:
+        loss, _, ubar = spectrum_loss(p, traj[-1], target)
     acc = _zero_partials()
     for n in range(nsteps - 1, -1, -1):
         ubar, part = step_vjp(p, traj[n], ubar, tb)
         for k in SIGMA_PARTIALS:
             acc[k] += part[k]
     grads = [O.ifft(p, x) for x in ubar]
-    return loss, grads, sigma_gradient(p, acc), acc
+    return loss, grads, sigma_gradient(p, acc, dgam), acc

@@ -36,7 +36,7 @@ EXPORTS = (
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
     "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store", "chns_graphs_enable",
-    "chns_status", "chns_free_energy_noalloc",
+    "chns_status", "chns_free_energy_noalloc", "chns_adj_seed_free_energy",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -125,6 +125,7 @@ def load_library(path: str = LIB_PATH):
     lib.chns_graphs_enable.argtypes = [V, I]
     lib.chns_status.argtypes = [V, ctypes.POINTER(I)]
     lib.chns_free_energy_noalloc.argtypes = [V, D, D]
+    lib.chns_adj_seed_free_energy.argtypes = [V, D, D, D]
     _lib = lib
     return lib
 
@@ -318,6 +319,14 @@ class Context:
         self._check(self.lib.chns_adj_seed_spectrum(self.h, _dp(t), ctypes.byref(loss), _dp(S)),
                     "chns_adj_seed_spectrum")
         return loss.value, S
+
+    def adj_seed_free_energy(self):
+        """Loss = ephi + egphi of the truncated final phase fields and its seed.
+        Returns (loss, ephi, egphi)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.chns_adj_seed_free_energy(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
+                    "chns_adj_seed_free_energy")
+        return a.value, b.value, c.value
 
     def adj_seed(self, wbar=None, c1bar=None, c2bar=None):
         """Seed with spectral adjoints of the final state (IDXC complex, None = zero)."""

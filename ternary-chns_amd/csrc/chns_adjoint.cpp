@@ -30,6 +30,7 @@ struct chns_adj_state {
   double* dS = nullptr;          // [nshell]
   int* nbq = nullptr;            // [ADJ_NQ]
   int mb = 0, nsh = 0;
+  double dgam[3] = {0.0, 0.0, 0.0};   // direct dL/dgamma_i of the loss (the free-energy seed)
   // forward intermediates of every stage of a replayed segment (store mode):
   // slot 2k holds passes A-C of N(u_k), slot 2k+1 those of N(a_k)
   int store = 1;                 // 1: keep them when device memory allows, 0: recompute
@@ -235,11 +236,13 @@ void sigma_gradient(const chns_ctx* c, const double* q, double* out) {
   using namespace chns;
   const double rf = q[AQ_FAC] * 2.0 / eps;
   const double dg1 = q[AQ_G1] - q[AQ_G1I] / (g1 * g1) + 0.75 * eps * q[AQ_MC1] -
-                     g3 / (g1 * g1) * (q[AQ_A11] + q[AQ_A21]) - lam / (g1 * g1) * q[AQ_LG1] + rf * dd1;
+                     g3 / (g1 * g1) * (q[AQ_A11] + q[AQ_A21]) - lam / (g1 * g1) * q[AQ_LG1] + rf * dd1 +
+                     c->adj->dgam[0];
   const double dg2 = q[AQ_G2] - q[AQ_G2I] / (g2 * g2) + 0.75 * eps * q[AQ_MC2] -
-                     g3 / (g2 * g2) * (q[AQ_A12] + q[AQ_A22]) - lam / (g2 * g2) * q[AQ_LG2] + rf * dd2;
+                     g3 / (g2 * g2) * (q[AQ_A12] + q[AQ_A22]) - lam / (g2 * g2) * q[AQ_LG2] + rf * dd2 +
+                     c->adj->dgam[1];
   const double dg3 = q[AQ_G3] - q[AQ_G3I] / (g3 * g3) + (q[AQ_A11] + q[AQ_A21]) / g1 +
-                     (q[AQ_A12] + q[AQ_A22]) / g2 + rf * dd3;
+                     (q[AQ_A12] + q[AQ_A22]) / g2 + rf * dd3 + c->adj->dgam[2];
   // gamma_1 = s12 - s23 + s13, gamma_2 = s12 - s13 + s23, gamma_3 = s23 - s12 + s13
   out[0] = dg1 + dg2 - dg3;    // sigma_12
   out[1] = dg1 - dg2 + dg3;    // sigma_13
@@ -365,12 +368,54 @@ extern "C" int chns_adj_seed_spectrum(chns_ctx* c, const double* target, double*
   }
   if (loss) *loss = L;
   if (sph1) std::memcpy(sph1, S.data(), a->nsh * sizeof(double));
+  for (double& x : a->dgam) x = 0.0;   // the spectrum does not depend on sigma directly
   AHIP(c, hipMemcpyAsync(a->dS, d.data(), a->nsh * sizeof(double), hipMemcpyHostToDevice, c->stream));
   chns::cd* ubp[3] = {a->ub, a->ub + a->S, a->ub + 2 * a->S};
   ALAUNCH(c, chns::launch_adj_seed_spectrum(c->b.C1, ubp, a->dS, c->b.amask, c->d.N, c->d.factor_x,
                                             2.0 * M_PI / c->p.lx, a->nsh, c->stream));
   if ((rc = after_seed(c))) return rc;
   AHIP(c, hipStreamSynchronize(c->stream));
+  return CHNS_OK;
+}
+
+extern "C" int chns_adj_seed_free_energy(chns_ctx* c, double* loss, double* ephi, double* egphi) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  chns_adj_state* a = c->adj;
+  if (!a->forward_done) return chns_fail(c, CHNS_E_STATE, "chns_adj_forward first");
+  if ((rc = chns_diag_prepare(c))) return rc;
+  const int N = c->d.N, L = c->d.log2n;
+  const double g1 = c->d.g1, g2 = c->d.g2, g3 = c->d.g3, eps = c->d.eps;
+  const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
+  // T(c1), T(c2) and their gradients in real space (every read of the state
+  // inside the step is truncated; so is this one)
+  double* f[6];
+  if ((rc = chns_fe_fields(c, true, f))) return rc;
+  double q[chns::RED_Q];
+  if ((rc = chns_fe_sums(c, f, g1, g2, g3, q))) return rc;
+  const double e_bulk = q[0] * (6.0 / eps) * scale;     // free_energy.c:246-248
+  const double e_grad = q[1] * (0.375 * eps * scale);   // free_energy.c:335
+  // gamma_i weights a sum of its own in both energies: dL/dgamma_i directly
+  for (int k = 0; k < 3; ++k) {
+    if ((rc = chns_fe_sums(c, f, k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0, q))) return rc;
+    a->dgam[k] = q[0] * (6.0 / eps) * scale + q[1] * (0.375 * eps * scale);
+  }
+  // real-space adjoints in place, their r2c (truncated to the kept modes),
+  // then c_i bar = X_c - i kx X_x - i ky X_y
+  ALAUNCH(c, chns::launch_fe_adjoint(f, N, g1, g2, g3, (6.0 / eps) * scale, 0.375 * eps * scale, c->stream));
+  chns::cd* X[6] = {a->ab, a->ab + a->S, a->ab + 2 * a->S, a->gi, a->gi + a->S, a->gi + 2 * a->S};
+  for (int k = 0; k < 3; ++k) {
+    ALAUNCH(c, chns::launch_row_r2c(L, f[2 * k], f[2 * k + 1], c->dT1, c->dT2, c->b.tw, c->ldF, c->stream));
+    ALAUNCH(c, chns::launch_col_fwd(L, true, c->dT1, X[2 * k], c->b.tw, c->b.amask, nullptr, c->ldF, c->stream));
+    ALAUNCH(c, chns::launch_col_fwd(L, true, c->dT2, X[2 * k + 1], c->b.tw, c->b.amask, nullptr, c->ldF, c->stream));
+  }
+  chns::cd* ubp[3] = {a->ub, a->ub + a->S, a->ub + 2 * a->S};
+  ALAUNCH(c, chns::launch_fe_seed(X, ubp, N, c->d.factor_x, c->stream));
+  if ((rc = after_seed(c))) return rc;
+  AHIP(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = e_bulk + e_grad;
+  if (ephi) *ephi = e_bulk;
+  if (egphi) *egphi = e_grad;
   return CHNS_OK;
 }
 
@@ -381,6 +426,7 @@ extern "C" int chns_adj_seed(chns_ctx* c, const double* wbar, const double* c1ba
   if (!a->forward_done) return chns_fail(c, CHNS_E_STATE, "chns_adj_forward first");
   const int N = c->d.N, H = c->d.H;
   const double* src[3] = {wbar, c1bar, c2bar};
+  for (double& x : a->dgam) x = 0.0;   // a caller-given seed has no direct sigma dependence
   if ((rc = chns_stage_buffer(c))) return rc;
   for (int f = 0; f < 3; ++f) {
     chns::cd* dst = a->ub + f * a->S;

@@ -55,6 +55,18 @@ int launch_adj_seed_spectrum(const cd* C1, cd* const* Ub, const double* dS, cons
   return (int)hipGetLastError();
 }
 
+int launch_fe_adjoint(double* const* f, int N, double g1, double g2, double g3, double b, double q, hipStream_t s) {
+  const size_t n = (size_t)N * N;
+  k_fe_adjoint<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(f[0], f[1], f[2], f[3], f[4], f[5], n, g1, g2, g3, b, q);
+  return (int)hipGetLastError();
+}
+
+int launch_fe_seed(cd* const* X, cd* const* Ub, int N, double f, hipStream_t s) {
+  const size_t n = (size_t)N * (N / 2 + 1);
+  k_fe_seed<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(X[0], X[1], X[2], X[3], X[4], X[5], Ub[0], Ub[1], Ub[2], N, f);
+  return (int)hipGetLastError();
+}
+
 int launch_adj_accum(const double* part, int nslot, size_t slot_stride, int mb, const int* nbq, double* acc,
                      hipStream_t s) {
   k_adj_accum<<<ADJ_NQ, 256, 0, s>>>(part, nslot, slot_stride, mb, nbq, acc);

@@ -581,6 +581,49 @@ static __global__ void k_adj_seed_spectrum(const cd* __restrict__ C1, cd* Ub0, c
   Ub2[e] = mk(0.0, 0.0);
 }
 
+// Real-space adjoint of the ternary free energy ephi + egphi
+// (free_energy.c:238-248,317-335) of the fields f = c1, c2, d/dx c1, d/dy c1,
+// d/dx c2, d/dy c2 (packed [N][N] each), overwritten in place by
+// dL/dc1, dL/dc2, dL/d(d/dx c1), ...: with s = c1 + c2 = 1 - c3,
+// ephi = b sum g1 c1^2 (1-c1)^2 + g2 c2^2 (1-c2)^2 + g3 s^2 (1-s)^2 and
+// egphi = q sum g1 |grad c1|^2 + g2 |grad c2|^2 + g3 |grad c1 + grad c2|^2,
+// b = 6 scale/eps, q = 0.375 eps scale.  grid: ceil(N*N / 256)
+static __global__ void k_fe_adjoint(double* c1, double* c2, double* c1x, double* c1y, double* c2x, double* c2y,
+                                    size_t n, double g1, double g2, double g3, double b, double q) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double a1 = c1[p], a2 = c2[p], sm = a1 + a2;
+  const double d3 = g3 * 2.0 * sm * (1.0 - sm) * (1.0 - 2.0 * sm);   // d/ds of g3 s^2 (1-s)^2
+  c1[p] = b * (g1 * 2.0 * a1 * (1.0 - a1) * (1.0 - 2.0 * a1) + d3);
+  c2[p] = b * (g2 * 2.0 * a2 * (1.0 - a2) * (1.0 - 2.0 * a2) + d3);
+  const double sx = c1x[p] + c2x[p], sy = c1y[p] + c2y[p];
+  const double u1x = c1x[p], u1y = c1y[p], u2x = c2x[p], u2y = c2y[p];
+  c1x[p] = q * 2.0 * (g1 * u1x + g3 * sx);
+  c1y[p] = q * 2.0 * (g1 * u1y + g3 * sy);
+  c2x[p] = q * 2.0 * (g2 * u2x + g3 * sx);
+  c2y[p] = q * 2.0 * (g2 * u2y + g3 * sy);
+}
+
+// Spectral seed from the r2c of those adjoints (X[k], column-major, already
+// truncated): c_i bar = X_c - i kx X_x - i ky X_y (the transpose of the
+// multipliers i kx, i ky is their conjugate); omega bar = 0.
+// grid: ceil(Ny2*N / 256)
+static __global__ void k_fe_seed(const cd* __restrict__ X1, const cd* __restrict__ X2,
+                                 const cd* __restrict__ X1x, const cd* __restrict__ X1y,
+                                 const cd* __restrict__ X2x, const cd* __restrict__ X2y, cd* Ub0, cd* Ub1, cd* Ub2,
+                                 int N, double f) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int H = N / 2 + 1;
+  if (e >= (size_t)N * H) return;
+  const int j = (int)(e / N), i = (int)(e % N);
+  const double kx = f * (double)(i <= N / 2 ? i : i - N), ky = f * (double)j;
+  // (-i k) z = (k Im z, -k Re z)
+  const cd a = X1x[e], bb = X1y[e], c = X2x[e], d = X2y[e];
+  Ub0[e] = mk(0.0, 0.0);
+  Ub1[e] = mk(X1[e].x + kx * a.y + ky * bb.y, X1[e].y - kx * a.x - ky * bb.x);
+  Ub2[e] = mk(X2[e].x + kx * c.y + ky * d.y, X2[e].y - kx * c.x - ky * d.x);
+}
+
 // acc[q] += sum over slots and workgroups of part[slot][q][b] (b < nb[q]),
 // in a fixed order.  grid: ADJ_NQ workgroups of 256 threads.
 static __global__ void k_adj_accum(const double* __restrict__ part, int nslot, size_t slot_stride, int mb,

@@ -79,6 +79,41 @@ int reduce(chns_ctx* c, int kind, const double* const* f, double* q) {
 
 }  // namespace
 
+int chns_diag_prepare(chns_ctx* c) { return prepare(c); }
+
+int chns_fe_fields(chns_ctx* c, bool trunc, double** f) {
+  int rc;
+  if (c->nph == 2) {
+    // c1, c2 and their gradients (free_energy.c:218-221, 301-313)
+    if ((rc = real_pair(c, 1, chns::COP_ID, 2, chns::COP_ID, trunc, 0, 1)) ||
+        (rc = real_pair(c, 1, chns::COP_IKX, 1, chns::COP_IKY, trunc, 2, 3)) ||
+        (rc = real_pair(c, 2, chns::COP_IKX, 2, chns::COP_IKY, trunc, 4, 5)))
+      return rc;
+    for (int i = 0; i < 6; ++i) f[i] = real_buf(c, i);
+  } else {
+    if ((rc = real_pair(c, 1, chns::COP_ID, 1, chns::COP_IKX, trunc, 0, 1)) ||
+        (rc = real_pair(c, 1, chns::COP_IKY, -1, 0, trunc, 2, -1)))
+      return rc;
+    for (int i = 0; i < 3; ++i) f[i] = real_buf(c, i);
+    for (int i = 3; i < 6; ++i) f[i] = nullptr;
+  }
+  return 0;
+}
+
+int chns_fe_sums(chns_ctx* c, const double* const* f, double g1, double g2, double g3, double* q) {
+  chns::RedArgs A{};
+  for (int i = 0; i < 6; ++i) A.f[i] = f[i];
+  A.g1 = g1;
+  A.g2 = g2;
+  A.g3 = g3;
+  A.N = c->d.N;
+  double* out = c->dred + (size_t)chns::reduce_blocks(c->d.N) * chns::RED_Q;
+  DLAUNCH(c, chns::launch_reduce(c->nph == 2 ? chns::RED_FE_TER : chns::RED_FE_BIN, A, c->dred, out, c->stream));
+  DHIP(c, hipMemcpyAsync(q, out, chns::RED_Q * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DHIP(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 extern "C" int chns_free_energy(chns_ctx* c, double* ephi, double* egphi) {
   int rc = prepare(c);
   if (rc) return rc;
@@ -87,22 +122,13 @@ extern "C" int chns_free_energy(chns_ctx* c, double* ephi, double* egphi) {
   const double eps = c->d.eps;
   double q[chns::RED_Q];
   double e_bulk = 0.0, e_grad = 0.0;
+  double* f[6];
+  if ((rc = chns_fe_fields(c, false, f))) return rc;   // the untruncated fields
+  if ((rc = chns_fe_sums(c, f, c->d.g1, c->d.g2, c->d.g3, q))) return rc;
   if (c->nph == 2) {
-    // untruncated c1, c2 and their gradients (free_energy.c:218-221, 301-313)
-    if ((rc = real_pair(c, 1, chns::COP_ID, 2, chns::COP_ID, false, 0, 1)) ||
-        (rc = real_pair(c, 1, chns::COP_IKX, 1, chns::COP_IKY, false, 2, 3)) ||
-        (rc = real_pair(c, 2, chns::COP_IKX, 2, chns::COP_IKY, false, 4, 5)))
-      return rc;
-    const double* f[6] = {real_buf(c, 0), real_buf(c, 1), real_buf(c, 2), real_buf(c, 3), real_buf(c, 4), real_buf(c, 5)};
-    if ((rc = reduce(c, chns::RED_FE_TER, f, q))) return rc;
     e_bulk = q[0] * (6.0 / eps) * scale;                        // free_energy.c:246-248
     e_grad = q[1] * (0.375 * eps * scale);                      // free_energy.c:335
   } else {
-    if ((rc = real_pair(c, 1, chns::COP_ID, 1, chns::COP_IKX, false, 0, 1)) ||
-        (rc = real_pair(c, 1, chns::COP_IKY, -1, 0, false, 2, -1)))
-      return rc;
-    const double* f[6] = {real_buf(c, 0), real_buf(c, 1), real_buf(c, 2), nullptr, nullptr, nullptr};
-    if ((rc = reduce(c, chns::RED_FE_BIN, f, q))) return rc;
     e_bulk = q[0] * (12.0 * (c->p.sigma / eps)) * scale;                // free_energy.c:233-236
     e_grad = q[1] * ((0.75 * c->p.sigma * eps) * scale);                // free_energy.c:290
   }

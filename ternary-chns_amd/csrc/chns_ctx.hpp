@@ -58,6 +58,15 @@ bool chns_graphs_default();
 // packed real [N][N] c2r (scaled 1/(Nx*Ny)) of a column-major spectral field S
 // of this context, copied to host memory out (synchronises)
 int chns_real_of(chns_ctx* c, const chns::cd* S, double* out);
+// diagnostics scratch (dT1, dT2, dred), allocated on first use
+int chns_diag_prepare(chns_ctx* c);
+// the real fields of the free energy in the row-pass buffers: c1, c2, d/dx c1,
+// d/dy c1, d/dx c2, d/dy c2 (binary: c, d/dx c, d/dy c, then nulls), of the
+// state or of its truncation
+int chns_fe_fields(chns_ctx* c, bool trunc, double** f);
+// the two sums of free_energy.c (bulk, gradient) over those fields with the
+// weights g1..g3 (ternary), unscaled, into q[0..1] (synchronises)
+int chns_fe_sums(chns_ctx* c, const double* const* f, double g1, double g2, double g3, double* q);
 // frees the adjoint storage (chns_destroy, chns_adj_release)
 void chns_adj_free(chns_ctx* c);
 // device address of chns_free_energy_noalloc's static scratch array on `dev`

@@ -119,3 +119,65 @@ def test_sigma_partials_planted_error_is_caught():
           - L(replace(p, sigma_12=p.sigma_12 - h), nsteps, target)(*f)) / (2 * h)
     assert abs(fd - gsig[0]) <= 1e-6 * abs(fd)
     assert abs(fd - gbad[0]) > 1e-3 * abs(fd), name
+
+
+def fe_loss_fn(p, nsteps):
+    """Total free energy of the truncated phase fields after nsteps (the
+    reference's free_energy.c sums, evaluated on T(c))."""
+    def f(wz, c1, c2):
+        s = O.run(p, O.state_from_real(p, wz, c1, c2), nsteps)
+        t = O.State(s.cwz, O.truncation(p, s.cphi1), O.truncation(p, s.cphi2))
+        return sum(O.free_energy(p, t))
+    return f
+
+
+def test_free_energy_loss_value_and_seed():
+    """The loss equals the oracle's free_energy() of the truncated state, and
+    its seed gives the central difference along a random state direction."""
+    p = params()
+    s = O.state_from_real(p, *fields(p))
+    loss, ephi, egphi, ubar, _ = OA.free_energy_loss(p, s)
+    t = O.State(s.cwz, O.truncation(p, s.cphi1), O.truncation(p, s.cphi2))
+    e1, e2 = O.free_energy(p, t)
+    assert math.isclose(ephi, e1, rel_tol=1e-13) and math.isclose(egphi, e2, rel_tol=1e-13)
+    rng = np.random.default_rng(4)
+    d = [O.truncation(p, O.dfft(rng.standard_normal((p.Nx, p.Ny)))) for _ in range(2)]
+    h = 1e-5
+
+    def L(sign):
+        q = O.State(s.cwz, s.cphi1 + sign * h * d[0], s.cphi2 + sign * h * d[1])
+        return OA.free_energy_loss(p, q)[0]
+
+    fd = (L(1) - L(-1)) / (2 * h)
+    ad = OA.spec_dot(p, ubar[1], d[0]) + OA.spec_dot(p, ubar[2], d[1])
+    assert abs(fd - ad) <= 1e-8 * abs(ad)
+
+
+@pytest.mark.parametrize("N,nsteps", [(16, 3), (32, 2)])
+def test_free_energy_gradient_matches_central_differences(N, nsteps):
+    """Gradients of the final free energy with respect to the initial fields
+    and sigma_12/13/23 (whose direct part, the gamma_i weights of the energy
+    itself, a dropped term would miss: checked by planting it below)."""
+    p = params(N)
+    f = fields(p)
+    loss, grads, gsig, acc = OA.loss_and_grad(p, O.state_from_real(p, *f), nsteps, None)
+    L = fe_loss_fn(p, nsteps)
+    assert math.isclose(loss, L(*f), rel_tol=1e-13)
+    rng = np.random.default_rng(12)
+    for k in range(3):
+        d = rng.standard_normal((p.Nx, p.Ny))
+        h = 1e-4 if k else 1e-3
+        plus = [x.copy() for x in f]
+        minus = [x.copy() for x in f]
+        plus[k] += h * d
+        minus[k] -= h * d
+        fd = (L(*plus) - L(*minus)) / (2 * h)
+        ad = float(np.sum(grads[k] * d))
+        assert abs(fd - ad) <= 1e-6 * abs(ad), (k, fd, ad)
+    no_direct = OA.sigma_gradient(p, acc)   # without the loss's own gamma dependence
+    for k, name in enumerate(("sigma_12", "sigma_13", "sigma_23")):
+        h = 1e-4 * getattr(p, name)
+        fd = (fe_loss_fn(replace(p, **{name: getattr(p, name) + h}), nsteps)(*f)
+              - fe_loss_fn(replace(p, **{name: getattr(p, name) - h}), nsteps)(*f)) / (2 * h)
+        assert abs(fd - gsig[k]) <= 1e-6 * abs(gsig[k]), (name, fd, gsig[k])
+        assert abs(fd - no_direct[k]) > 1e-3 * abs(fd)

@@ -254,3 +254,66 @@ def test_stored_intermediates_match_recompute(N):
         scale = np.linalg.norm(outs[1][1]) + np.linalg.norm(outs[1][2])
         assert np.linalg.norm(outs[0][0] - outs[1][0]) < TOL * scale
         assert np.all(np.abs(outs[0][3] - outs[1][3]) <= TOL * np.abs(outs[1][3]))
+
+
+@pytest.mark.parametrize("N,nsteps,K", [(16, 5, 2), (32, 7, 3), (64, 4, 4)])
+def test_free_energy_loss_gradient_matches_oracle(N, nsteps, K):
+    """Final free energy (of the truncated phase fields) as the loss: value,
+    gradients for the initial fields and sigma (with its direct part) against
+    the oracle adjoint (oracle/chns_adjoint.py, pinned by finite differences of
+    the oracle forward in tests/test_adjoint_oracle.py)."""
+    op, cp, _, reals = adj_case(N)
+    loss, grads, gsig, _ = OA.loss_and_grad(op, O.state_from_real(op, *reals), nsteps, None)
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(*reals)
+        ctx.adj_forward(nsteps, K)
+        gl, e1, e2 = ctx.adj_seed_free_energy()
+        gw, g1, g2, gs = ctx.adj_backward()
+        state = ctx.get_state()
+    assert abs(gl - loss) <= 1e-12 * loss and abs(e1 + e2 - gl) <= 1e-15 * gl
+    for a, b in zip((gw, g1, g2), grads):
+        assert rel(a, b) < TOL
+    assert np.all(np.abs(gs - gsig) <= 1e-9 * np.abs(gsig))
+    ref = O.run(op, O.state_from_real(op, *reals), nsteps)
+    for a, b in zip(state, ref.fields()):
+        assert rel(a, b) < 1e-12                      # the seed leaves the state as it was
+
+
+def gpu_fe_loss(cp, reals, nsteps):
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(*reals)
+        ctx.step(nsteps)
+        ctx.adj_forward(0, 1)
+        return ctx.adj_seed_free_energy()[0]
+
+
+@pytest.mark.parametrize("N,nsteps,K", [(128, 12, 5)])
+def test_free_energy_gradient_matches_central_differences(N, nsteps, K):
+    """The library's free-energy gradient against central differences of the
+    library forward, for c1, c2 and sigma_12/13/23."""
+    from dataclasses import replace
+
+    op, cp, kw, reals = adj_case(N)
+    with C.Context(cp) as ctx:
+        ctx.set_real_state(*reals)
+        ctx.adj_forward(nsteps, K)
+        ctx.adj_seed_free_energy()
+        gw, g1, g2, gs = ctx.adj_backward()
+    rng = np.random.default_rng(9)
+    for k, g in ((1, g1), (2, g2)):
+        d = rng.standard_normal((N, N))
+
+        def f(h):
+            moved = [x.copy() for x in reals]
+            moved[k] += h * d
+            return gpu_fe_loss(cp, moved, nsteps)
+
+        fd = richardson(f, 1e-5)
+        ad = float(np.sum(g * d))
+        assert abs(fd - ad) <= TOL_FD * abs(ad), (k, fd, ad)
+    for i, name in enumerate(("sigma_12", "sigma_13", "sigma_23")):
+        def f(h):
+            return gpu_fe_loss(C.params("ternary", **{**kw, name: kw[name] * (1.0 + h)}), reals, nsteps)
+
+        fd = richardson(f, 1e-5) / kw[name]
+        assert abs(fd - gs[i]) <= TOL_FD * abs(gs[i]), (name, fd, gs[i])
