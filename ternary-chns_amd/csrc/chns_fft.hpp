@@ -274,7 +274,16 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
 // transpose (decimation in time): it takes x in that layout and returns
 // X[t + 256 j].  So spectral data keeps the cyclic layout of fft() while
 // real-space points sit at w4_pos(t, j) (see below).
-constexpr int W4_Q = 512 + (512 >> CHNS_LDS_PAD);   // LDS entries per wave quarter: >= lp(511) + 1
+constexpr int W4_Q = 512;   // LDS entries per wave quarter
+
+// LDS index of entry i of a quarter: the low 3 bits XORed with bits 3-5, no
+// padding.  Sixteen consecutive entries (the reads of every exchange) stay in
+// sixteen distinct 16-byte bank groups, and so do eight entries at stride 8
+// (the first stage's writes); the one-in-eight padding of lp() costs the
+// consecutive reads a two-way conflict (measured on MI355X: 127 instead of
+// 245 B/clk per CU for 16-byte reads; 16-byte writes run at ~76 B/clk either
+// way).
+__device__ __forceinline__ int lw(int i) { return i ^ ((i >> 3) & 7); }
 
 __device__ __forceinline__ int w4_pos(int t, int j) { return 4 * (t & 63) + (t >> 6) + 256 * j; }
 
@@ -353,10 +362,10 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const W4Tw<HELD>& W, 
     const int base = l * 8;
     cd w1 = W.template get<INV>(2, t), w2 = W.template get<INV>(3, t), w4 = W.template get<INV>(4, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) lq[lp(base + q)] = a[q];
+    for (int q = 0; q < 8; ++q) lq[lw(base + q)] = a[q];
     wave_sync();
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = lq[lp(l + 64 * m)];
+    for (int m = 0; m < 8; ++m) v[m] = lq[lw(l + 64 * m)];
     wave_sync();
     // stage NS = 8: k = l % 8, twiddle exponent k * 8 (W_512), table index 4 k 8
     a[0] = v[0];
@@ -375,10 +384,10 @@ __device__ __forceinline__ void fft512_wave(cd* v, cd* lq, const W4Tw<HELD>& W, 
     w2 = W.template get<INV>(6, t);
     w4 = W.template get<INV>(7, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) lq[lp(b2 + q * 8)] = a[q];
+    for (int q = 0; q < 8; ++q) lq[lw(b2 + q * 8)] = a[q];
     wave_sync();
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = lq[lp(l + 64 * m)];
+    for (int m = 0; m < 8; ++m) v[m] = lq[lw(l + 64 * m)];
     wave_sync();
     // stage NS = 64: k = l, twiddle exponent l (W_512), table index 4 l; last stage
     v[1] = cmulc(v[1], w1);
@@ -419,16 +428,16 @@ __device__ __forceinline__ void fft_w4_inv(cd* v, cd* lds, const W4Tw<HELD>& W, 
       wm2 = rot90<true>(w2t);
     }
     const cd wm3 = cmulc(wm1, wm2);
-    lds[lp(m)] = a0;
-    lds[W4_Q + lp(m)] = cmulc(a1, wm1);
-    lds[2 * W4_Q + lp(m)] = cmulc(a2, wm2);
-    lds[3 * W4_Q + lp(m)] = cmulc(a3, wm3);
+    lds[lw(m)] = a0;
+    lds[W4_Q + lw(m)] = cmulc(a1, wm1);
+    lds[2 * W4_Q + lw(m)] = cmulc(a2, wm2);
+    lds[3 * W4_Q + lw(m)] = cmulc(a3, wm3);
   }
   lds_sync();
   const int q = t >> 6, l = t & 63;
   cd* lq = lds + q * W4_Q;
 #pragma unroll
-  for (int j = 0; j < 8; ++j) v[j] = lq[lp(l + 64 * j)];
+  for (int j = 0; j < 8; ++j) v[j] = lq[lw(l + 64 * j)];
   wave_sync();
   fft512_wave<true, false>(v, lq, W, t);
 }
@@ -443,7 +452,7 @@ __device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const W4Tw<HELD>& W, 
   fft512_wave<false, false>(v, lq, W, t);
   wave_sync();
 #pragma unroll
-  for (int j = 0; j < 8; ++j) lq[lp(l + 64 * j)] = v[j];
+  for (int j = 0; j < 8; ++j) lq[lw(l + 64 * j)] = v[j];
   const cd w1t = W.template get<false>(0, t), w2t = W.template get<false>(1, t);
   const cd c1 = mk(kR, -kR);   // W_2048^256 = exp(-i pi/4)
   lds_sync();
@@ -456,10 +465,10 @@ __device__ __forceinline__ void fft_w4_fwd(cd* v, cd* lds, const W4Tw<HELD>& W, 
       wk2 = rot90<false>(w2t);
     }
     const cd wk3 = cmulc(wk1, wk2);
-    cd b0 = lds[lp(k)];
-    cd b1 = cmulc(lds[W4_Q + lp(k)], wk1);
-    cd b2 = cmulc(lds[2 * W4_Q + lp(k)], wk2);
-    cd b3 = cmulc(lds[3 * W4_Q + lp(k)], wk3);
+    cd b0 = lds[lw(k)];
+    cd b1 = cmulc(lds[W4_Q + lw(k)], wk1);
+    cd b2 = cmulc(lds[2 * W4_Q + lw(k)], wk2);
+    cd b3 = cmulc(lds[3 * W4_Q + lw(k)], wk3);
     dft4<false>(b0, b1, b2, b3);
     v[jp] = b0;
     v[jp + 2] = b1;
@@ -494,7 +503,7 @@ __device__ __forceinline__ void fft_w4_fwd_split(cd* v, cd* lds, const W4Tw<HELD
   fft512_wave<false, false>(v, lq, W, t);
   wave_sync();
 #pragma unroll
-  for (int j = 0; j < 8; ++j) lq[lp(l + 64 * j)] = v[j];
+  for (int j = 0; j < 8; ++j) lq[lw(l + 64 * j)] = v[j];
   const cd w1t = W.template get<false>(0, t), w2t = W.template get<false>(1, t);
   lds_sync();
   const bool t0 = t == 0;
@@ -502,20 +511,20 @@ __device__ __forceinline__ void fft_w4_fwd_split(cd* v, cd* lds, const W4Tw<HELD
   cd x[4], y[4];
   {
     const cd w3 = cmulc(w1t, w2t);
-    x[0] = lds[lp(t)];
-    x[1] = cmulc(lds[W4_Q + lp(t)], w1t);
-    x[2] = cmulc(lds[2 * W4_Q + lp(t)], w2t);
-    x[3] = cmulc(lds[3 * W4_Q + lp(t)], w3);
+    x[0] = lds[lw(t)];
+    x[1] = cmulc(lds[W4_Q + lw(t)], w1t);
+    x[2] = cmulc(lds[2 * W4_Q + lw(t)], w2t);
+    x[3] = cmulc(lds[3 * W4_Q + lw(t)], w3);
     dft4<false>(x[0], x[1], x[2], x[3]);   // X[t + 512 q]
   }
   {
     const cd u1 = t0 ? mk(kR, -kR) : mk(-w1t.y, -w1t.x);     // W^256 = exp(-i pi/4)
     const cd u2 = t0 ? mk(0.0, -1.0) : mk(-w2t.x, w2t.y);    // W^512 = -i
     const cd u3 = cmulc(u1, u2);
-    y[0] = lds[lp(k2)];
-    y[1] = cmulc(lds[W4_Q + lp(k2)], u1);
-    y[2] = cmulc(lds[2 * W4_Q + lp(k2)], u2);
-    y[3] = cmulc(lds[3 * W4_Q + lp(k2)], u3);
+    y[0] = lds[lw(k2)];
+    y[1] = cmulc(lds[W4_Q + lw(k2)], u1);
+    y[2] = cmulc(lds[2 * W4_Q + lw(k2)], u2);
+    y[3] = cmulc(lds[3 * W4_Q + lw(k2)], u3);
     dft4<false>(y[0], y[1], y[2], y[3]);   // X[k2 + 512 q]
   }
   cd A, B;
