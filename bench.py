@@ -212,9 +212,11 @@ def ensemble_members():
     return out
 
 
-def ensemble_bench(steps: int, warmup: int):
+def ensemble_bench(steps: int, warmup: int, legs: str = "both"):
     """Speedup of one graph-launched batched ensemble over the same instances run
-    one after another, each alone on the GPU (sum of their timed steps)."""
+    one after another, each alone on the GPU (sum of their timed steps).
+    legs="batch" runs the batched leg only (no per-member contexts are created
+    and destroyed; used for profiling runs)."""
     import numpy as np
 
     import chns_amd as C
@@ -222,7 +224,7 @@ def ensemble_bench(steps: int, warmup: int):
     members = ensemble_members()
     ps = [C.params(m, **kw) for m, kw, _ in members]
     seq = 0.0
-    for p, (m, kw, f) in zip(ps, members):
+    for p, (m, kw, f) in zip(ps, members) if legs == "both" else ():
         with C.Context(p) as ctx:
             ctx.set_real_state(None, *f)
             ctx.step(effective_warmup(warmup))
@@ -250,7 +252,7 @@ def ensemble_bench(steps: int, warmup: int):
     pts = sum(kw["Nx"] * kw["Ny"] for _, kw, _ in members)
     return {
         "metric": "batched ensemble speedup over sequential single-instance runs",
-        "value": seq / batch,
+        "value": seq / batch if legs == "both" else None,
         "unit": "x",
         "n_gpus": 1,
         "steps": steps,
@@ -264,6 +266,7 @@ def ensemble_bench(steps: int, warmup: int):
         "config": {"workload": "ensemble 32x256^2 + 16x512^2 + 4x1024^2, binary/ternary alternating, "
                                "sigma ~ U[0.05,0.15], dt=5e-4*(256/N)^3 (ternary x (min|gamma|/0.1)^2), HIP graph replays of 8 steps",
                    "instances": len(members), "groups": groups, "parallelism": "single GPU"},
+        "graph_replays": os.environ.get("CHNS_NO_GRAPH", "") != "1",
         "sequential_s": seq,
         "batch_s": {"untuned": res[False][0], "tuned": res[True][0]},
         "tuned_splits": res[True][1],
@@ -344,6 +347,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--traffic-pass", default="rowD", help="pass whose HBM traffic is counted")
     ap.add_argument("--ensemble", action="store_true", help="measure the batched-ensemble workload instead")
+    ap.add_argument("--ensemble-legs", choices=("both", "batch"), default="both",
+                    help="ensemble: both legs (the speedup), or the batched leg only")
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="adjoint leg: forward steps (0: skip)")
     ap.add_argument("--adjoint-ckpt", type=int, default=100, help="adjoint leg: checkpoint interval")
     a = ap.parse_args()
@@ -354,7 +359,7 @@ def main():
         if a.gpus != 1:
             print("bench.py: the ensemble runs on one GPU", file=sys.stderr)
             sys.exit(2)
-        print(json.dumps(ensemble_bench(a.steps if a.steps != 200 else 500, a.warmup)))
+        print(json.dumps(ensemble_bench(a.steps if a.steps != 200 else 500, a.warmup, a.ensemble_legs)))
         return
     if a.gpus != 1:
         print(f"bench.py: measures one GPU only (got --gpus {a.gpus}); "
