@@ -60,12 +60,13 @@ def effective_warmup(requested: int) -> int:
 def kernel_tag(pass_name: str, N: int) -> str:
     """Substring of the demangled kernel name of a pass (ternary model)."""
     L = int(round(math.log2(N)))
-    # single-instance launches of the plain step: the last template argument is
-    # the SMOOTH_FILTER switch (ensembles use chns::InstBatch)
-    inst = "chns::Inst, false>"
-    return {"colA": f"k_colA<{L}, {inst}", "rowB": f"k_rowB<{L}, 2, {inst}", "colC": f"k_colC<{L}, 2, {inst}",
-            "rowD": f"k_rowD<{L}, 2, {inst}", "colE1": f"k_colE<{L}, 2, 1, {inst}",
-            "colE2": f"k_colE<{L}, 2, 2, {inst}"}[pass_name]
+    # single-instance launches of the plain step: then the SMOOTH_FILTER switch
+    # and the default FFT schedule (rows per workgroup 0, points per thread 8);
+    # ensembles use chns::InstBatch
+    row, col = "chns::Inst, false, 0>", "chns::Inst, false, 8>"
+    return {"colA": f"k_colA<{L}, chns::Inst, false>", "rowB": f"k_rowB<{L}, 2, {row}",
+            "colC": f"k_colC<{L}, 2, {col}", "rowD": f"k_rowD<{L}, 2, {row}",
+            "colE1": f"k_colE<{L}, 2, 1, {col}", "colE2": f"k_colE<{L}, 2, 2, {col}"}[pass_name]
 
 
 def measure_traffic(N: int, tag: str, timeout: int = 120):
@@ -244,7 +245,7 @@ def ensemble_bench(steps: int, warmup: int, legs: str = "both"):
             t0 = time.perf_counter()
             ens.step(steps)
             ens.sync()
-            res[tuned] = (time.perf_counter() - t0, splits)
+            res[tuned] = (time.perf_counter() - t0, splits, ens.schedule())
         finite = all(bool(np.all(np.isfinite(x))) for b in range(len(ens)) for x in ens[b].get_state())
         groups = ens.groups()
     best = min(res, key=lambda k: res[k][0])
@@ -270,6 +271,7 @@ def ensemble_bench(steps: int, warmup: int, legs: str = "both"):
         "sequential_s": seq,
         "batch_s": {"untuned": res[False][0], "tuned": res[True][0]},
         "tuned_splits": res[True][1],
+        "tuned_schedules": res[True][2],
         "grid_pt_steps_per_s": pts * steps / batch,
         "finite": finite,
     }

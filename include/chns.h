@@ -205,11 +205,24 @@ chns_ctx *chns_ensemble_instance(chns_ensemble *e, int b);
  * queued; chns_ensemble_sync waits. */
 int chns_ensemble_step(chns_ensemble *e, int nsteps);
 int chns_ensemble_sync(chns_ensemble *e);
-/* Autotune the launch split of every group (1, 2 or 4 concurrent slices),
- * timing `reps` replays of the 8-step graph per candidate, largest groups
- * first; the instances' states are restored afterwards.  splits (may be
- * NULL) receives the choice per group, in group order. */
+/* FFT schedules of a group's batched passes: bit 0 runs the column passes
+ * with 16 points per thread (radix-16 stages, one LDS exchange fewer per
+ * 2048-point transform) instead of 8, bit 1 gives the row passes half the
+ * rows per workgroup.  0 is the single-context schedule.  All schedules give
+ * the same results to round-off. */
+#define CHNS_ENSEMBLE_NSCHED 4
+/* Autotune every group (largest grids first): for each FFT schedule that
+ * differs from the default at the group's grid size and each launch split
+ * (1, 2 or 4 concurrent slices), time `reps` replays of the 8-step graph,
+ * and keep the fastest; the instances' states are restored afterwards.
+ * splits (may be NULL) receives the chosen split per group, in group order;
+ * chns_ensemble_schedule gives both choices. */
 int chns_ensemble_tune(chns_ensemble *e, int reps, int *splits);
+/* Current split and FFT schedule per group (arrays may be NULL); returns the
+ * number of groups. */
+int chns_ensemble_schedule(const chns_ensemble *e, int *split, int *sched);
+/* Set a group's split and schedule by hand (waits for queued work). */
+int chns_ensemble_set_schedule(chns_ensemble *e, int group, int split, int sched);
 /* Groups in order: log2 N, model and instance count (arrays may be NULL);
  * returns the number of groups. */
 int chns_ensemble_groups(const chns_ensemble *e, int *log2n, int *model, int *count);

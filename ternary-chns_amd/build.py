@@ -49,6 +49,7 @@ def build(verbose: bool = False) -> str:
         "adjdev": os.path.join(BUILD, "chns_adjoint.o"),
         "filter": os.path.join(BUILD, "chns_filter.o"),
         "noalloc": os.path.join(BUILD, "chns_noalloc.o"),
+        "sched": os.path.join(BUILD, "chns_batch_sched.o"),
     }
     steps = [
         ["g++", "-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-I" + INC,
@@ -71,11 +72,13 @@ def build(verbose: bool = False) -> str:
          "-c", os.path.join(CSRC, "chns_filter.hip"), "-o", objs["filter"]],
         [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
          "-c", os.path.join(CSRC, "chns_noalloc.hip"), "-o", objs["noalloc"]],
+        [HIPCC, "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-I" + INC, *NO_PK,
+         "-c", os.path.join(CSRC, "chns_batch_sched.hip"), "-o", objs["sched"]],
     ]
     with ThreadPoolExecutor(8) as ex:
         list(ex.map(_run, steps))
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", OUT, objs["device"], objs["batch"],
-          objs["adjdev"], objs["filter"], objs["noalloc"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
+          objs["adjdev"], objs["filter"], objs["noalloc"], objs["sched"], objs["api"], objs["diag"], objs["ensemble"], objs["adjapi"], objs["tables"]])
     # the reference-named entry points (chns_ref.h) over the C ABI, and the
     # time-loop driver on them (plain C), found next to the library via rpath
     _run(["gcc", "-O2", "-std=c11", "-D_GNU_SOURCE", "-Wall", "-Wextra", "-fPIC", "-shared", "-I" + INC,

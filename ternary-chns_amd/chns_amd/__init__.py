@@ -36,7 +36,8 @@ EXPORTS = (
     "chns_ensemble_size", "chns_ensemble_instance", "chns_ensemble_step", "chns_ensemble_sync",
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
     "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store", "chns_graphs_enable",
-    "chns_status", "chns_free_energy_noalloc", "chns_adj_seed_free_energy",
+    "chns_status", "chns_free_energy_noalloc", "chns_adj_seed_free_energy", "chns_ensemble_schedule",
+    "chns_ensemble_set_schedule",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -126,6 +127,8 @@ def load_library(path: str = LIB_PATH):
     lib.chns_status.argtypes = [V, ctypes.POINTER(I)]
     lib.chns_free_energy_noalloc.argtypes = [V, D, D]
     lib.chns_adj_seed_free_energy.argtypes = [V, D, D, D]
+    lib.chns_ensemble_schedule.argtypes = [V, ctypes.POINTER(I), ctypes.POINTER(I)]
+    lib.chns_ensemble_set_schedule.argtypes = [V, I, I, I]
     _lib = lib
     return lib
 
@@ -410,11 +413,22 @@ class Ensemble:
         return [(a[k], b[k], c[k]) for k in range(n)]
 
     def tune(self, reps: int = 5):
-        """Autotune the launch split per group; returns the chosen splits."""
+        """Autotune the launch split and FFT schedule per group; returns the chosen splits."""
         n = len(self.groups())
         out = (ctypes.c_int * n)()
         self._check(self.lib.chns_ensemble_tune(self.h, int(reps), out), "chns_ensemble_tune")
         return list(out)
+
+    def schedule(self):
+        """[(split, sched)] per group (sched bit 0: radix-16 column passes, bit 1: half-row workgroups)."""
+        n = len(self.groups())
+        a, b = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+        self.lib.chns_ensemble_schedule(self.h, a, b)
+        return list(zip(list(a), list(b)))
+
+    def set_schedule(self, group: int, split: int, sched: int):
+        self._check(self.lib.chns_ensemble_set_schedule(self.h, int(group), int(split), int(sched)),
+                    "chns_ensemble_set_schedule")
 
     def close(self):
         if getattr(self, "h", None):

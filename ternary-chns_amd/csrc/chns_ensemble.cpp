@@ -29,6 +29,7 @@ struct chns_ensemble {
   };
   struct Group {
     int log2n = 0, model = 0, nph = 0, Jt = 0, split = 1;
+    int sched = 0;              // FFT schedule of the batched passes (chns_batch.hip)
     std::vector<int> members;   // instance indices
     chns::Bufs* dB = nullptr;   // device arrays, one entry per member
     chns::KParams* dP = nullptr;
@@ -104,13 +105,23 @@ int step_part(chns_ensemble* e, const chns_ensemble::Group& g, const chns_ensemb
   const chns::Bufs* B = g.dB + p.first;
   const chns::KParams* P = g.dP + p.first;
   const int L = g.log2n, M = g.model, nb = p.count;
+  const int sc = g.sched;
   for (int st = 1; st <= 2; ++st) {
-    if (g.nph > 0) ENS_LAUNCH(e, chns::launch_batch_rowB(L, M, B, P, nb, p.s));
-    ENS_LAUNCH(e, chns::launch_batch_colC(L, M, B, P, g.Jt, nb, p.s));
-    ENS_LAUNCH(e, chns::launch_batch_rowD(L, M, B, P, nb, p.s));
-    ENS_LAUNCH(e, chns::launch_batch_colE(L, M, st, B, P, 1 + g.nph, nb, p.s));
+    if (g.nph > 0) ENS_LAUNCH(e, chns::launch_batch_rowB(L, M, B, P, nb, sc, p.s));
+    ENS_LAUNCH(e, chns::launch_batch_colC(L, M, B, P, g.Jt, nb, sc, p.s));
+    ENS_LAUNCH(e, chns::launch_batch_rowD(L, M, B, P, nb, sc, p.s));
+    ENS_LAUNCH(e, chns::launch_batch_colE(L, M, st, B, P, 1 + g.nph, nb, sc, p.s));
   }
   return CHNS_OK;
+}
+
+// Schedules that differ from the default at grid size 2^log2n: the radix-16
+// column passes from N = 256 up (8 points per thread below), half the rows
+// per row workgroup below N = 2048 (one row per workgroup there).
+bool schedule_distinct(int log2n, int sched) {
+  if ((sched & 1) && log2n < 8) return false;
+  if ((sched & 2) && log2n >= 11) return false;
+  return sched >= 0 && sched < CHNS_ENSEMBLE_NSCHED;
 }
 
 // Steps per replay of the ensemble graph: as for chns_step, a graph of 8
@@ -298,9 +309,12 @@ extern "C" int chns_ensemble_sync(chns_ensemble* e) {
 
 extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
   // Greedy per group, largest grids first: time `reps` replays of the graph
-  // (kGraphSteps steps each) for each split of the group into 1, 2, 4
-  // concurrently launched slices and keep the fastest.  The instances' spectral state is saved before and restored
-  // after, so tuning leaves the physics untouched.
+  // (kGraphSteps steps each) for every FFT schedule of the group's batched
+  // passes that differs from the default at its size, each with the group
+  // split into 1, 2 and 4 concurrently launched slices, and keep the fastest
+  // pair (the other groups keep their current choice meanwhile).  The
+  // instances' spectral state is saved before and restored after, so tuning
+  // leaves the physics untouched.
   if (!e || reps < 1) return CHNS_E_ARG;
   ENS_HIP(e, hipSetDevice(e->dev));
   int rc = refresh_inputs(e);
@@ -348,20 +362,26 @@ extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
   };
   for (int gi : order) {
     auto& g = e->groups[gi];
-    int best = 1;
+    int best = 1, best_sched = 0;
     double best_ms = 1e300;
-    for (int split : {1, 2, 4}) {
-      if (split > (int)g.members.size()) break;
-      double ms = 0.0;
-      if ((rc = make_parts(e, g, split)) != CHNS_OK || (rc = time_graph(&ms)) != CHNS_OK) {
-        free_save();
-        return rc;
-      }
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = split;
+    for (int sched = 0; sched < CHNS_ENSEMBLE_NSCHED; ++sched) {
+      if (sched && !schedule_distinct(g.log2n, sched)) continue;
+      g.sched = sched;
+      for (int split : {1, 2, 4}) {
+        if (split > (int)g.members.size()) break;
+        double ms = 0.0;
+        if ((rc = make_parts(e, g, split)) != CHNS_OK || (rc = time_graph(&ms)) != CHNS_OK) {
+          free_save();
+          return rc;
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = split;
+          best_sched = sched;
+        }
       }
     }
+    g.sched = best_sched;
     if ((rc = make_parts(e, g, best)) != CHNS_OK) {
       free_save();
       return rc;
@@ -397,4 +417,26 @@ extern "C" int chns_ensemble_groups(const chns_ensemble* e, int* log2n, int* mod
     if (count) count[g] = (int)e->groups[g].members.size();
   }
   return (int)e->groups.size();
+}
+
+extern "C" int chns_ensemble_schedule(const chns_ensemble* e, int* split, int* sched) {
+  if (!e) return 0;
+  for (size_t g = 0; g < e->groups.size(); ++g) {
+    if (split) split[g] = e->groups[g].split;
+    if (sched) sched[g] = e->groups[g].sched;
+  }
+  return (int)e->groups.size();
+}
+
+extern "C" int chns_ensemble_set_schedule(chns_ensemble* e, int group, int split, int sched) {
+  if (!e) return CHNS_E_ARG;
+  if (group < 0 || group >= (int)e->groups.size()) return efail(e, CHNS_E_ARG, "no such group");
+  if (split < 1 || sched < 0 || sched >= CHNS_ENSEMBLE_NSCHED)
+    return efail(e, CHNS_E_ARG, "split must be >= 1 and sched in [0, CHNS_ENSEMBLE_NSCHED)");
+  ENS_HIP(e, hipSetDevice(e->dev));
+  ENS_HIP(e, hipStreamSynchronize(e->s0));   // the slices' streams are replaced
+  auto& g = e->groups[group];
+  g.sched = sched;
+  drop_graph(e);
+  return make_parts(e, g, split);
 }

@@ -58,13 +58,15 @@ template <int LOG2N, int PP = 8> struct ColCfg {
   static constexpr int C = (LOG2N >= 11 && P == 8) ? 2 : C0;
   static constexpr int NT = C * T;
 };
-// rows per row-kernel workgroup
-template <int LOG2N> struct RowCfg {
+// rows per row-kernel workgroup; RV = 1 (a schedule of the ensemble
+// autotuner) halves them (at N = 2048 a row workgroup holds one row either way)
+template <int LOG2N, int RV = 0> struct RowCfg {
   static constexpr int N = 1 << LOG2N;
   static constexpr int P = 8;
   static constexpr int T = N / P;
   static constexpr int R0 = (256 / T) > 1 ? (256 / T) : 1;
-  static constexpr int R = R0 < N ? R0 : N;
+  static constexpr int R1 = (R0 >> RV) > 1 ? (R0 >> RV) : 1;
+  static constexpr int R = R1 < N ? R1 : N;
   static constexpr int NT = R * T;
 };
 
@@ -86,8 +88,8 @@ __device__ __forceinline__ size_t tix(int x, int j, int ld) {
 // b, b+8, b+16, ... which sit on the same XCD (blocks are dealt to the 8 XCDs
 // round-robin) and start together: their partial-line writes of the outputs
 // merge in that L2, and the two rows of an input pair read one line.
-template <int LOG2N> __device__ __forceinline__ int row_of(int r) {
-  constexpr int R = RowCfg<LOG2N>::R;
+template <int LOG2N, int RV = 0> __device__ __forceinline__ int row_of(int r) {
+  constexpr int R = RowCfg<LOG2N, RV>::R;
   if constexpr (R >= TF) {
     return blockIdx.x * R + r;
   } else {
@@ -365,15 +367,15 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
 
 // ============================================================ pass B
 // grid: N / R
-template <int LOG2N, int MODEL, class Args, bool FILT = false>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
+template <int LOG2N, int MODEL, class Args, bool FILT = false, int RV = 0>
+__global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT)) void k_rowB(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
-  using RC = RowCfg<LOG2N>;
+  using RC = RowCfg<LOG2N, RV>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
-  const int x = row_of<LOG2N>(r);
+  const int x = row_of<LOG2N, RV>(r);
   cd* lds = lds_all + r;
   cd v[8];
   RowTw<LOG2N, RC::R> tws;
@@ -416,11 +418,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_rowB(Args args) {
 // grid: 1-D over (tile, group) items (col_item): ternary 6 groups, binary 4, NS 2.
 // Groups are dispatched in order, the one-transform groups (d/dx c_i) last,
 // so they fill the final round of workgroups.
-template <int LOG2N, int MODEL, class Args, bool FILT = false>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
+template <int LOG2N, int MODEL, class Args, bool FILT = false, int PP = 8>
+__global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
-  using CF = ColCfg<LOG2N>;
+  using CF = ColCfg<LOG2N, PP>;
   constexpr int N = CF::N, T = CF::T;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
@@ -502,15 +504,15 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colC(Args args) {
 
 // ============================================================ pass D
 // grid: N / R
-template <int LOG2N, int MODEL, class Args, bool FILT = false>
-__global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
+template <int LOG2N, int MODEL, class Args, bool FILT = false, int RV = 0>
+__global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT), 2) void k_rowD(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
-  using RC = RowCfg<LOG2N>;
+  using RC = RowCfg<LOG2N, RV>;
   constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
-  const int x = row_of<LOG2N>(r);
+  const int x = row_of<LOG2N, RV>(r);
   cd* lds = lds_all + r;
   const size_t ro = tix<TJ>(x, 0, P.ldJ);   // row bases in the tiled layouts: element k at +TJ*k
   const size_t fo = tix<TF>(x, 0, P.ldF);   // ... and at +TF*k
@@ -743,11 +745,11 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_rowD(Args args) {
 
 // ============================================================ pass E
 // grid: 1-D over (tile, field) items (col_item): omega, then phase field(s).
-template <int LOG2N, int MODEL, int STAGE, class Args, bool FILT = false>
-__global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_colE(Args args) {
+template <int LOG2N, int MODEL, int STAGE, class Args, bool FILT = false, int PP = 8>
+__global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
-  using CF = ColCfg<LOG2N>;
+  using CF = ColCfg<LOG2N, PP>;
   constexpr int N = CF::N, T = CF::T, H = N / 2 + 1;
   __shared__ cd lds_all[CF::C * LdsSize<LOG2N>::value];
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;

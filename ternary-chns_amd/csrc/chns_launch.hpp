@@ -33,13 +33,21 @@ int launch_filt_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipS
 int launch_filt_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nfields, hipStream_t s);
 
 // batched passes (chns_batch.hip): nb instances of one size and model, device
-// arrays B[nb], P[nb]
+// arrays B[nb], P[nb]; sched: the group's FFT schedule (bit 0: radix-16
+// column passes, bit 1: half the rows per row workgroup)
 int launch_batch_colA(int log2n, const Bufs* B, const KParams* P, int Jt, int nph, int nb, hipStream_t s);
-int launch_batch_rowB(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
-int launch_batch_colC(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, hipStream_t s);
-int launch_batch_rowD(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
-int launch_batch_colE(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb,
+int launch_batch_rowB(int log2n, int model, const Bufs* B, const KParams* P, int nb, int sched, hipStream_t s);
+int launch_batch_colC(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, int sched,
                       hipStream_t s);
+int launch_batch_rowD(int log2n, int model, const Bufs* B, const KParams* P, int nb, int sched, hipStream_t s);
+int launch_batch_colE(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb, int sched,
+                      hipStream_t s);
+// the schedule variants (chns_batch_sched.hip)
+int launch_batch_rowB_half(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
+int launch_batch_rowD_half(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s);
+int launch_batch_colC_r16(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, hipStream_t s);
+int launch_batch_colE_r16(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb,
+                          hipStream_t s);
 
 // diagnostics (chns_diag.hpp)
 struct RedArgs;

@@ -18,35 +18,41 @@ template <int L, bool FILT = false, class A> int colA(const A& a, int Jt, int np
   k_colA<L, A, FILT><<<dim3(ceil_div<L, 16>(Jt), nph, nb), ColCfg<L, 16>::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
-  const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
-  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else k_rowB<L, MODEL_BINARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
+template <int L, bool FILT = false, int RV = 0, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
+  using RC = RowCfg<L, RV>;
+  const dim3 g((1 << L) / RC::R, 1, nb);
+  if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  else k_rowB<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
-  const dim3 g(round16(ceil_div<L>(Jt) * groupsC(model)), 1, nb);
-  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else k_colC<L, MODEL_NS, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+template <int L, bool FILT = false, int PP = 8, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
+  const dim3 g(round16(ceil_div<L, PP>(Jt) * groupsC(model)), 1, nb);
+  constexpr int NT = ColCfg<L, PP>::NT;
+  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  else k_colC<L, MODEL_NS, A, FILT, PP><<<g, NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
-  const dim3 g((1 << L) / RowCfg<L>::R, 1, nb);
-  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
-  else k_rowD<L, MODEL_NS, A, FILT><<<g, RowCfg<L>::NT, 0, s>>>(a);
+template <int L, bool FILT = false, int RV = 0, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
+  using RC = RowCfg<L, RV>;
+  const dim3 g((1 << L) / RC::R, 1, nb);
+  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  else k_rowD<L, MODEL_NS, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, int ST, bool FILT = false, class A> int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
-  const dim3 g(round16(ceil_div<L>((1 << L) / 2 + 1) * nf), 1, nb);
-  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
-  else k_colE<L, MODEL_NS, ST, A, FILT><<<g, ColCfg<L>::NT, 0, s>>>(a);
+template <int L, int ST, bool FILT = false, int PP = 8, class A>
+int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
+  const dim3 g(round16(ceil_div<L, PP>((1 << L) / 2 + 1) * nf), 1, nb);
+  constexpr int NT = ColCfg<L, PP>::NT;
+  if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  else k_colE<L, MODEL_NS, ST, A, FILT, PP><<<g, NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, class A> int colE(int model, int stage, const A& a, int nf, int nb, hipStream_t s) {
-  return stage == 1 ? colE_st<L, 1, FILT>(model, a, nf, nb, s) : colE_st<L, 2, FILT>(model, a, nf, nb, s);
+template <int L, bool FILT = false, int PP = 8, class A>
+int colE(int model, int stage, const A& a, int nf, int nb, hipStream_t s) {
+  return stage == 1 ? colE_st<L, 1, FILT, PP>(model, a, nf, nb, s) : colE_st<L, 2, FILT, PP>(model, a, nf, nb, s);
 }
 
 }  // namespace passes

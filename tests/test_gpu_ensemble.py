@@ -163,6 +163,9 @@ def test_c4_workload(tuned):
         if tuned:
             splits = ens.tune(reps=1)
             assert all(s in (1, 2, 4) for s in splits)
+            sched = ens.schedule()
+            assert [s for s, _ in sched] == splits and all(0 <= c < 4 for _, c in sched)
+            print("tuned (split, schedule) per group:", list(zip(ens.groups(), sched)), flush=True)
         ens.step(3)
         early = {b: ens[b].get_state() for b in (1, 32, 49)}   # ternary members of 256^2, 512^2, 1024^2
         ens.step(9)
@@ -181,3 +184,41 @@ def test_c4_workload(tuned):
         op = O.Params(model=m, **kw)
         s3 = O.run(op, O.state_from_real(op, None, *f), 3)
         assert close(g, s3.fields(), 1e-10), b
+
+
+SCHED_MEMBERS = [("ternary", 256, 11), ("binary", 512, 12), ("ternary", 1024, 13), ("ns", 256, 14),
+                 ("binary", 64, 15), ("ternary", 512, 16)]
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2, 3])
+def test_every_fft_schedule_matches_single_contexts(sched):
+    """Each FFT schedule of the batched passes (bit 0: radix-16 column passes,
+    bit 1: half the rows per row workgroup), set on every group with a split
+    of 2 where it has two members, evolves each member as a stand-alone
+    context does (1e-13: the schedules differ in FFT radix, i.e. round-off)."""
+    ps, ops, states = build(SCHED_MEMBERS)
+    # smaller dt at the larger grids (the explicit k^2 muF rate grows as N^3 at eps = 1.5 dx)
+    for p in ps:
+        p.dt = 5e-4 * min(1.0, (64.0 / p.Nx) ** 3)
+    refs = [single(p, s, 9) for p, s in zip(ps, states)]
+    with C.Ensemble(ps) as ens:
+        for b, s in enumerate(states):
+            ens[b].set_state(*s.fields())
+        for g, (_, _, count) in enumerate(ens.groups()):
+            ens.set_schedule(g, min(2, count), sched)
+        assert all(sc == sched for _, sc in ens.schedule())
+        ens.step(9)
+        ens.sync()
+        got = [ens[b].get_state() for b in range(len(ps))]
+    for b in range(len(ps)):
+        assert all(np.all(np.isfinite(x)) for x in got[b])
+        assert close(got[b], refs[b], TOL_SAME), (b, sched)
+
+
+def test_schedule_rejections():
+    ps, _, _ = build(MEMBERS[:2])
+    with C.Ensemble(ps) as ens:
+        with pytest.raises(C.ChnsError):
+            ens.set_schedule(0, 1, 4)
+        with pytest.raises(C.ChnsError):
+            ens.set_schedule(len(ens.groups()), 1, 0)
