@@ -110,7 +110,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD1(AdjArgs args
   const AdjBufs& A = args.A;
   const KParams& P = args.P;
   using RC = RowCfg<LOG2N>;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   constexpr bool STG = AdjRowLoader<LOG2N>::STG;
   __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT, 2) void k_adj_rowD2(AdjArgs args
   const AdjBufs& A = args.A;
   const KParams& P = args.P;
   using RC = RowCfg<LOG2N>;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   constexpr bool STG = AdjRowLoader<LOG2N>::STG;
   __shared__ cd stg_all[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_adj_rowB(AdjArgs args) {
   const AdjBufs& A = args.A;
   const KParams& P = args.P;
   using RC = RowCfg<LOG2N>;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;

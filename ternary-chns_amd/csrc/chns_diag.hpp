@@ -61,7 +61,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_c2r_full(const cd* __
                                                                   int ldF, double scale) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T, H = N / 2 + 1;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;

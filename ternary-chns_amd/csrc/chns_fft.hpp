@@ -542,6 +542,12 @@ __device__ __forceinline__ void fft_w4_fwd_split(cd* v, cd* lds, const W4Tw<HELD
   }
 }
 
+// LDS entries of a row kernel's transform array: the four swizzled wave
+// quarters of the 2048-point transform, R padded rows otherwise.
+template <int LOG2N, int R> struct RowLds {
+  static constexpr int value = (LOG2N == 11 && R == 1) ? 4 * W4_Q : R * LdsSize<LOG2N>::value;
+};
+
 // Row transform of a kernel with R rows per workgroup (rows interleaved
 // across lanes, 8 points per thread): the 4-wave transform at N = 2048 with
 // one row per workgroup, fft() otherwise.  rpos gives the real-space index of

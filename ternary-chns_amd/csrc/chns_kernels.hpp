@@ -373,7 +373,7 @@ __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT)) void k_rowB(Args args) {
   const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N, RV>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
   const int x = row_of<LOG2N, RV>(r);
   cd* lds = lds_all + r;
@@ -510,7 +510,7 @@ __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT), 2) void k_rowD(Args args) 
   const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N, RV>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
   const int x = row_of<LOG2N, RV>(r);
   cd* lds = lds_all + r;
@@ -755,7 +755,25 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
   const int c = threadIdx.x % CF::C, t = threadIdx.x / CF::C;
   constexpr int NG = MODEL == MODEL_TERNARY ? 3 : (MODEL == MODEL_BINARY ? 2 : 1);
   int tile, g;
-  if (!col_item((H + CF::C - 1) / CF::C, NG, &tile, &g)) return;
+  // omega items first, then the phase-field items in pairs (c1 tile, c2 tile)
+  // on one XCD at one time, so the second reads the coefficient tables from L2
+  {
+    const int ntl = (H + CF::C - 1) / CF::C, nw = (ntl + 1) & ~1;
+    int item, dummy;
+    if (!col_item(nw + (NG - 1) * ntl, 1, &item, &dummy)) return;
+    if (item < nw) {
+      if (item >= ntl) return;
+      tile = item;
+      g = 0;
+    } else if constexpr (NG > 1) {
+      const int r = item - nw;
+      tile = r / (NG - 1);
+      g = 1 + r % (NG - 1);
+    } else {
+      return;
+    }
+  }
+
   const int j = tile * CF::C + c;
   const bool ok = j < H;
   cd* lds = lds_all + c;
@@ -852,7 +870,7 @@ __global__ __launch_bounds__(RowCfg<LOG2N>::NT) void k_row_r2c(const double* __r
                                                                const cd* __restrict__ tw, int ldF) {
   using RC = RowCfg<LOG2N>;
   constexpr int N = RC::N, T = RC::T;
-  __shared__ cd lds_all[RC::R * LdsSize<LOG2N>::value];
+  __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
   const int x = row_of<LOG2N>(r);
   cd* lds = lds_all + r;

@@ -43,7 +43,9 @@ template <int L, bool FILT = false, int RV = 0, class A> int rowD(int model, con
 }
 template <int L, int ST, bool FILT = false, int PP = 8, class A>
 int colE_st(int model, const A& a, int nf, int nb, hipStream_t s) {
-  const dim3 g(round16(ceil_div<L, PP>((1 << L) / 2 + 1) * nf), 1, nb);
+  // omega tiles (rounded up to an even count), then nf - 1 phase items per tile (k_colE)
+  const int ntl = ceil_div<L, PP>((1 << L) / 2 + 1);
+  const dim3 g(round16(((ntl + 1) & ~1) + (nf - 1) * ntl), 1, nb);
   constexpr int NT = ColCfg<L, PP>::NT;
   if (model == MODEL_TERNARY) k_colE<L, MODEL_TERNARY, ST, A, FILT, PP><<<g, NT, 0, s>>>(a);
   else if (model == MODEL_BINARY) k_colE<L, MODEL_BINARY, ST, A, FILT, PP><<<g, NT, 0, s>>>(a);
