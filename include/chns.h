@@ -148,6 +148,11 @@ int chns_free_energy_noalloc(chns_ctx *ctx, double *ephi, double *egphi);
  * the points with c_i >= 0.5 (kinetic_energy.c:3-144); eph2 is ternary only,
  * eph1/eph2 untouched for NS.  Any output may be NULL. */
 int chns_kinetic_energy(chns_ctx *ctx, double *eu, double *eph1, double *eph2);
+/* chns_kinetic_energy with the per-phase point counts n1, n2 (the points
+ * with c_i >= 0.5: count, count1, count2 of kinetic_energy.c:55-80), which
+ * decide whether the reference writes its BINARY line (kinetic_energy.c:
+ * 103-108 prints only when count != 0).  Any output may be NULL. */
+int chns_kinetic_energy_counts(chns_ctx *ctx, double *eu, double *eph1, double *eph2, double *n1, double *n2);
 /* Interface counts of the phase fields thresholded at 0.5 (perimeter.c:127-238):
  * counts[0] sign changes to the x+1/y+1 neighbour and counts[1] points inside
  * for phase 1, counts[2..3] for phase 2 (ternary).  The reference prints

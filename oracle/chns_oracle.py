@@ -580,7 +580,9 @@ def main_loop(p: Params, s: State, maxiter: int, nfile: int, diag_every: int = 1
     def diagnostics(s):
         if nph:
             rec["fe"].append((t,) + tuple(free_energy(p, s)))
-        rec["ken"].append((t,) + tuple(kinetic_energy(p, s)))
+        ke, cnts = kinetic_energy(p, s, counts=True)
+        if nph != 1 or cnts[0] != 0:                                 # binary: kinetic_energy.c:103-108
+            rec["ken"].append((t,) + tuple(ke))
         if nph:
             for ph, (c1, c2) in enumerate(perimeter(p, s)):
                 spt = np.float64(c1 * p.dx)
@@ -652,11 +654,12 @@ def free_energy(p: Params, s: State):
     return ephi, egphi
 
 
-def kinetic_energy(p: Params, s: State):
+def kinetic_energy(p: Params, s: State, counts: bool = False):
     """Mean KE and per-phase KE (kinetic_energy.c:3-144).
 
     Returns (eu,) for NS, (eu, eph) binary, (eu, eph1, eph2) ternary, with the
-    per-phase values already divided by their counts (when non-zero).
+    per-phase values already divided by their counts (when non-zero); with
+    ``counts``, also the list of those counts (points with c_i >= 0.5).
     """
     cux, cuy = get_velocity(p, s.cwz)
     ux = ifft(p, truncation(p, cux))
@@ -665,15 +668,16 @@ def kinetic_energy(p: Params, s: State):
     e = 0.5 * (ux * ux + uy * uy)
     eu = float(np.sum(e)) * scale
     if p.model == MODEL_NS:
-        return (eu,)
-    outs = [eu]
+        return ((eu,), []) if counts else (eu,)
+    outs, cnts = [eu], []
     for cphi in ([s.cphi1] if p.model == MODEL_BINARY else [s.cphi1, s.cphi2]):
         ph = ifft(p, truncation(p, cphi))
         m = ph >= 0.5
         cnt = int(np.sum(m))
         eph = float(np.sum(np.where(m, e, 0.0)))
         outs.append(eph / cnt if cnt != 0 else eph)
-    return tuple(outs)
+        cnts.append(cnt)
+    return (tuple(outs), cnts) if counts else tuple(outs)
 
 
 def spectrum(p: Params, s: State):

@@ -37,7 +37,7 @@ EXPORTS = (
     "chns_ensemble_tune", "chns_ensemble_groups", "chns_adj_forward", "chns_adj_seed_spectrum", "chns_adj_seed",
     "chns_adj_backward", "chns_adj_get", "chns_adj_release", "chns_adj_set_store", "chns_graphs_enable",
     "chns_status", "chns_free_energy_noalloc", "chns_adj_seed_free_energy", "chns_ensemble_schedule",
-    "chns_ensemble_set_schedule",
+    "chns_ensemble_set_schedule", "chns_kinetic_energy_counts",
 )
 # coefficient partial sums behind the sigma gradient (chns_adj_get)
 ADJ_PARTIALS = ("fac", "G1", "G2", "G3", "g1i", "g2i", "g3i", "mc1", "mc2",
@@ -112,6 +112,7 @@ def load_library(path: str = LIB_PATH):
     lib.chns_ensemble_groups.argtypes = [V, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(I)]
     lib.chns_free_energy.argtypes = [V, D, D]
     lib.chns_kinetic_energy.argtypes = [V, D, D, D]
+    lib.chns_kinetic_energy_counts.argtypes = [V, D, D, D, D, D]
     lib.chns_perimeter.argtypes = [V, D]
     lib.chns_spectrum.argtypes = [V, I, D, D, D, D, D, D, D]
     lib.chns_real_field.argtypes = [V, I, I, D]
@@ -273,6 +274,16 @@ class Context:
         self._check(self.lib.chns_kinetic_energy(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)),
                     "chns_kinetic_energy")
         return (a.value, b.value, c.value)[: 1 + self.nph]
+
+    def kinetic_energy_counts(self):
+        """(kinetic_energy(), [n1[, n2]]): the energies and the per-phase point
+        counts (c_i >= 0.5) they are averaged over (kinetic_energy.c:62-79)."""
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n1, n2 = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.chns_kinetic_energy_counts(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                                        ctypes.byref(n1), ctypes.byref(n2)),
+                    "chns_kinetic_energy_counts")
+        return (a.value, b.value, c.value)[: 1 + self.nph], [int(n1.value), int(n2.value)][: self.nph]
 
     def perimeter(self):
         """[(interface, inside) per phase field] (perimeter.c:127-238)."""

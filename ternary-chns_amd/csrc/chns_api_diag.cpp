@@ -139,7 +139,8 @@ extern "C" int chns_free_energy(chns_ctx* c, double* ephi, double* egphi) {
   return CHNS_OK;
 }
 
-extern "C" int chns_kinetic_energy(chns_ctx* c, double* eu, double* eph1, double* eph2) {
+extern "C" int chns_kinetic_energy_counts(chns_ctx* c, double* eu, double* eph1, double* eph2, double* n1,
+                                         double* n2) {
   int rc = prepare(c);
   if (rc) return rc;
   const double scale = 1.0 / ((double)c->p.Nx * c->p.Ny);
@@ -155,7 +156,13 @@ extern "C" int chns_kinetic_energy(chns_ctx* c, double* eu, double* eph1, double
   if (eu) *eu = q[0] * scale;                                   // kinetic_energy.c:95
   if (eph1) *eph1 = q[2] != 0.0 ? q[1] / q[2] : q[1];           // kinetic_energy.c:105-107,124-127
   if (eph2) *eph2 = q[4] != 0.0 ? q[3] / q[4] : q[3];           // kinetic_energy.c:130-133
+  if (n1) *n1 = c->nph >= 1 ? q[2] : 0.0;                       // count / count1
+  if (n2) *n2 = c->nph == 2 ? q[4] : 0.0;                       // count2
   return CHNS_OK;
+}
+
+extern "C" int chns_kinetic_energy(chns_ctx* c, double* eu, double* eph1, double* eph2) {
+  return chns_kinetic_energy_counts(c, eu, eph1, eph2, nullptr, nullptr);
 }
 
 extern "C" int chns_perimeter(chns_ctx* c, double* counts) {

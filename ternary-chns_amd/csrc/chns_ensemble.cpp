@@ -317,6 +317,8 @@ extern "C" int chns_ensemble_tune(chns_ensemble* e, int reps, int* splits) {
   // leaves the physics untouched.
   if (!e || reps < 1) return CHNS_E_ARG;
   ENS_HIP(e, hipSetDevice(e->dev));
+  // queued replays finish before their graph and slice streams are replaced
+  ENS_HIP(e, hipStreamSynchronize(e->s0));
   int rc = refresh_inputs(e);
   if (rc != CHNS_OK) return rc;
   std::vector<void*> save;

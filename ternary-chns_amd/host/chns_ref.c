@@ -122,11 +122,13 @@ void free_energy(double (*cphi1)[2], double (*cphi2)[2]) {
 
 void kinetic_energy(double (*cwz)[2], double (*cphi1)[2], double (*cphi2)[2]) {
   upload(cwz, cphi1, cphi2);
-  double eu = 0, e1 = 0, e2 = 0;
-  CK(chns_kinetic_energy(g.ctx, &eu, &e1, &e2));
+  double eu = 0, e1 = 0, e2 = 0, n1 = 0;
+  CK(chns_kinetic_energy_counts(g.ctx, &eu, &e1, &e2, &n1, NULL));
   if (!g.fu) return;
   if (g.nph == 2) fprintf(g.fu, "%lf \t %.12lf \t %.12lf \t %.12lf\n", g.t, eu, e1, e2);   /* :135 */
-  else if (g.nph == 1) fprintf(g.fu, "%lf \t %.12lf \t %.12lf\n", g.t, eu, e1);             /* :109 */
+  else if (g.nph == 1) {
+    if (n1 != 0) fprintf(g.fu, "%lf \t %.12lf \t %.12lf\n", g.t, eu, e1);                 /* :103-109 */
+  }
   else fprintf(g.fu, "%lf \t %.12lf\n", g.t, eu);                                          /* :141 */
 }
 

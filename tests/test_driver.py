@@ -181,3 +181,27 @@ def test_driver_matches_oracle_loop(tmp_path, case):
             got = read_raw(tmp_path / f"{name}.{fl}", p.Nx)
             scale = max(np.max(np.abs(ref)), 1e-300)
             assert np.max(np.abs(got - ref)) <= 1e-10 * scale + 1e-300, (name, fl)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mirror", [0, 1])
+def test_binary_line_skipped_without_phase_points(tmp_path, mirror):
+    """BINARY kinetic_energy() writes its line only when some point has
+    c >= 0.5 (kinetic_energy.c:103-108): a read-in field below 0.5 everywhere
+    leaves data/ken_t.dat empty while the other files keep their lines, and
+    the oracle's loop records no row either."""
+    N, maxiter, nfile = 32, 4, 2
+    rng = np.random.default_rng(7)
+    c = 0.3 + 0.05 * rng.random((N, N))
+    (tmp_path / "in").mkdir()
+    c.tofile(tmp_path / "in" / "c.in")
+    print(f"chns2d binary read, all c < 0.5, mirror={mirror}", flush=True)
+    r = run(tmp_path, model="binary", Nx=N, init="read", read1=tmp_path / "in" / "c.in", maxiter=maxiter,
+            nfile=nfile, mirror=mirror)
+    assert r.returncode == 0, r.stderr
+    d = tmp_path / "data"
+    assert (d / "ken_t.dat").stat().st_size == 0
+    assert len(_cols(d / "free_en_t.dat")) == maxiter + 1
+    p = O.Params(Nx=N, Ny=N, model="binary")
+    rec, _ = O.main_loop(p, O.state_from_real(p, None, c), maxiter, nfile)
+    assert rec["ken"] == [] and len(rec["fe"]) == maxiter + 1

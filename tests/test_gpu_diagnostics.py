@@ -47,8 +47,11 @@ def test_kinetic_energy_and_dump(model, N):
     with C.Context(cp) as ctx:
         ctx.set_state(*s.fields())
         ke = ctx.kinetic_energy()
+        ke2, counts = ctx.kinetic_energy_counts()
         fields = [ctx.real_field(k, True) for k in range(1 + ctx.nph)]
     assert relerr(ke, O.kinetic_energy(op, s)) < TOL
+    ref_ke, ref_counts = O.kinetic_energy(op, s, counts=True)
+    assert ke2 == ke and counts == ref_counts                   # the point counts are exact
     for f, ref in zip(fields, s.fields()):
         assert O.rel_l2(f, O.ifft(op, O.truncation(op, ref))) < 1e-14
 
