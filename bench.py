@@ -142,7 +142,7 @@ def host_cpu():
     return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(N, kw, warm=2, steps=20, max_seconds=30.0, threads=None):
+def cpu_baseline(N, kw, warm=2, steps=150, max_seconds=30.0, threads=None):
     """The C/OpenMP restatement of the step (oracle/chns_step.c: the same
     reference functions as the numpy oracle, its own radix-2 FFT, two real
     fields per complex transform) on the host cores: ``warm`` untimed steps,
