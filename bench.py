@@ -121,9 +121,11 @@ def pass_bytes(N, Jt, K):
         "rowB": 2 * Ft + 2 * F,
         "colC": 10 * Ft + 3 * Fk,
         "rowD": 10 * Ft + 4 * F,
-        "colE1": 15 * F + 2 * Ft,
-        # stage 2 reads the saved a - z2 N(u) and not the state: 3F less than the
+        # stage 1 writes the intermediate state a only on its kept modes (the only
+        # ones read before stage 2 overwrites it), and stage 2 reads the saved
+        # a - z2 N(u) instead of the state: together 6F - 3Fk less per step than the
         # five-pass budget B_alg = 42F + 48Ft + 6Fk, which stays the reported figure
+        "colE1": 12 * F + 2 * Ft + 3 * Fk,
         "colE2": 12 * F + 2 * Ft,
     }, 42 * F + 48 * Ft + 6 * Fk
 

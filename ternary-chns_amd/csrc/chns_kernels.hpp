@@ -821,6 +821,14 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
   // fin: every updated mode of this thread is finite (re + im is finite
   // unless one part is inf or NaN, or both overflow together)
   bool fin = true;
+  // Stage 1 in place: only the kept modes of a are ever read (pass C of stage
+  // 2 reads T(a), pass A is fused below, stage 2 forms u' from b), so the
+  // others are not written.  Written in full when the unpruned kernels read
+  // every mode (FILT), for the buoyancy term (k_gravity reads the
+  // untruncated state) and when writing out of place (the adjoint's replay
+  // keeps a).
+  const bool fullA = FILT || P.grav || So != S;
+  const int keptA = fullA ? N : (j < P.Jt ? B.amask[j] : -1);
   if (ok) {
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
@@ -833,7 +841,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
         const cd k1 = (g == 0 ? B.tabw1 : B.tabc1)[ti];
         const double e = k1.x, z = k1.y;
         const cd a = cadd(cscale(u0, e), cscale(v[m], z));
-        So[idx] = a;
+        if (absidx<LOG2N>(i) <= keptA) So[idx] = a;
         Nsv[idx] = csub(a, cscale(v[m], z2));
         s[m] = a;
         fin = fin && __builtin_isfinite(a.x + a.y);
