@@ -205,3 +205,22 @@ def test_binary_line_skipped_without_phase_points(tmp_path, mirror):
     p = O.Params(Nx=N, Ny=N, model="binary")
     rec, _ = O.main_loop(p, O.state_from_real(p, None, c), maxiter, nfile)
     assert rec["ken"] == [] and len(rec["fe"]) == maxiter + 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mirror", [0, 1])
+def test_free_energy_guard_exits_3(tmp_path, mirror):
+    """free_energy.c:151-154: the run stops once ephi >= 100.  A BINARY run
+    with dt far above the explicit k^2 P(c) limit diverges within a few steps;
+    the driver (free_energy() of chns_ref.h) writes that step's free-energy
+    line, then exits with status 3 and says why on stderr."""
+    N = 32
+    print(f"chns2d binary droplet, dt = 5 (diverges), mirror={mirror}", flush=True)
+    r = run(tmp_path, model="binary", Nx=N, init="droplet", dt=5.0, maxiter=50, nfile=10, mirror=mirror)
+    assert r.returncode == 3, (r.returncode, r.stderr)
+    assert "free_energy" in r.stderr
+    lines = (tmp_path / "data" / "free_en_t.dat").read_text().splitlines()
+    assert 2 <= len(lines) <= 51
+    last = float(lines[-1].split()[1])
+    assert not (last < 100.0)              # the line that tripped the guard (or nan)
+    assert all(float(x.split()[1]) < 100.0 for x in lines[:-1])
