@@ -144,13 +144,14 @@ def host_cpu():
     return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
 
 
-def cpu_baseline(N, kw, warm=2, steps=150, max_seconds=30.0, threads=None):
+def cpu_baseline(N, kw, warm=2, steps=150, max_seconds=30.0, threads=None, fields=None, what="lens"):
     """The C/OpenMP restatement of the step (oracle/chns_step.c: the same
     reference functions as the numpy oracle, its own radix-2 FFT, two real
     fields per complex transform) on the host cores: ``warm`` untimed steps,
     then up to ``steps`` timed ones (fewer if ``max_seconds`` runs out), on the
-    lens workload of the GPU leg.  ``threads``: the cores this process may use,
-    at most 16 (one GPU's share of the host)."""
+    workload of the GPU leg (``fields``: its real initial c1, c2; the lens by
+    default).  ``threads``: the cores this process may use, at most 16 (one
+    GPU's share of the host)."""
     import oracle as O
     from oracle import c_oracle as CO
 
@@ -158,7 +159,7 @@ def cpu_baseline(N, kw, warm=2, steps=150, max_seconds=30.0, threads=None):
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
     used = CO.threads(threads)
     op = O.Params(model="ternary", **kw)
-    s0 = O.state_from_real(op, None, *O.init_lens_real(op))
+    s0 = O.state_from_real(op, None, *(fields if fields is not None else O.init_lens_real(op)))
     st = CO.Stepper(op)
     try:
         st.set_state(s0)
@@ -176,8 +177,46 @@ def cpu_baseline(N, kw, warm=2, steps=150, max_seconds=30.0, threads=None):
     cpu = host_cpu()
     return {"value": N * N * done / dt, "unit": "grid-pt steps/s", "cores": used, "kind": "port",
             "cpu_model": cpu["model"], "nproc": cpu["nproc"], "affinity": cpu["affinity"],
-            "sample": f"{done} ternary ETD2RK steps of the {N}^2 lens workload after {warm} warm-up steps "
+            "sample": f"{done} ternary ETD2RK steps of the {N}^2 {what} workload after {warm} warm-up steps "
                       f"(C/OpenMP restatement oracle/chns_step.c, {used} threads), {dt:.1f} s"}
+
+
+def forced_ternary_1024():
+    """The forced ternary workload at 1024^2: c1, c2 = 1/3 + U[-0.05, 0.05]
+    (SplitMix64 seed 2), all sigma_ij = 0.1, eps = 1.5 dx, lambda = 1e-3,
+    nu = 5e-3, Xlambda = 2, Kolmogorov forcing 0.1 sin(4y) on omega; dt = 5e-4
+    diverges at this size (the explicit k^2 muF rate grows as N^3 at eps = 1.5 dx),
+    so dt = 5e-4 (256/1024)^3; the timing does not depend on dt."""
+    from chns_amd import ic
+
+    N, L = 1024, 2.0 * math.pi
+    kw = dict(Nx=N, Ny=N, lx=L, ly=L, nu=5e-3, dt=5e-4 * (256.0 / N) ** 3, lam=1e-3, epsilon=1.5 * L / N,
+              sigma_12=0.1, sigma_13=0.1, sigma_23=0.1, Xlambda=2.0, f_amp=0.1, f_k=4)
+    return kw, ic.noise(N, N, 2, 1.0 / 3.0, 0.05, 2)
+
+
+def forced_ternary_leg(steps=200, warmup=64, cpu=True):
+    """Step throughput of the forced ternary 1024^2 workload on the GPU (after
+    its own warm-up) and, with ``cpu``, the C/OpenMP restatement beside it."""
+    import chns_amd as C
+
+    kw, (c1, c2) = forced_ternary_1024()
+    N = kw["Nx"]
+    with C.Context(C.params("ternary", **kw), device=0) as ctx:
+        ctx.set_real_state(None, c1, c2)
+        ctx.step(warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+    out = {"workload": f"ternary {N}^2, c = 1/3 + U[-0.05, 0.05] (seed 2), all sigma_ij = 0.1, Xlambda = 2, "
+                       f"forcing 0.1 sin(4y), dt = {kw['dt']:.3g}",
+           "steps": steps, "ms_per_step": dt * 1e3 / steps, "value": N * N * steps / dt, "unit": "grid-pt steps/s"}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(N, kw, steps=100, max_seconds=15.0, fields=(c1, c2),
+                                           what="forced ternary")
+    return out
 
 
 def ensemble_members():
@@ -347,7 +386,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=2048)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU baselines (and the forced ternary 1024^2 leg they are timed beside)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs")
     ap.add_argument("--traffic-pass", default="rowD", help="pass whose HBM traffic is counted")
     ap.add_argument("--ensemble", action="store_true", help="measure the batched-ensemble workload instead")
@@ -450,6 +490,9 @@ def main():
     }
     if a.adjoint_steps > 0:
         out["adjoint"] = adjoint_bench(N, a.adjoint_steps, a.adjoint_ckpt)
+    if not a.no_cpu_baseline:
+        if N == 2048:
+            out["forced_ternary_1024"] = forced_ternary_leg()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, kw)
     print(json.dumps(out))
