@@ -61,11 +61,13 @@ def kernel_tag(pass_name: str, N: int) -> str:
     """Substring of the demangled kernel name of a pass (ternary model)."""
     L = int(round(math.log2(N)))
     # single-instance launches of the plain step: then the SMOOTH_FILTER switch
-    # and the default FFT schedule (rows per workgroup 0, points per thread 8);
+    # and the default FFT schedule (rows per workgroup 0, points per thread 8),
+    # passes C and D with the conservative vorticity advection (true);
     # ensembles use chns::InstBatch
     row, col = "chns::Inst, false, 0>", "chns::Inst, false, 8>"
+    rowc, colc = "chns::Inst, false, 0, true>", "chns::Inst, false, 8, true>"
     return {"colA": f"k_colA<{L}, chns::Inst, false>", "rowB": f"k_rowB<{L}, 2, {row}",
-            "colC": f"k_colC<{L}, 2, {col}", "rowD": f"k_rowD<{L}, 2, {row}",
+            "colC": f"k_colC<{L}, 2, {colc}", "rowD": f"k_rowD<{L}, 2, {rowc}",
             "colE1": f"k_colE<{L}, 2, 1, {col}", "colE2": f"k_colE<{L}, 2, 2, {col}"}[pass_name]
 
 
@@ -119,12 +121,15 @@ def pass_bytes(N, Jt, K):
     return {
         "colA": 2 * Fk + 2 * Ft,
         "rowB": 2 * Ft + 2 * F,
-        "colC": 10 * Ft + 3 * Fk,
-        "rowD": 10 * Ft + 4 * F,
+        # the conservative vorticity advection: pass C writes T(w) instead of
+        # T(i kx w), T(i ky w), and pass D reads it (one Ft less in each)
+        "colC": 9 * Ft + 3 * Fk,
+        "rowD": 9 * Ft + 4 * F,
         # stage 1 writes the intermediate state a only on its kept modes (the only
         # ones read before stage 2 overwrites it), and stage 2 reads the saved
-        # a - z2 N(u) instead of the state: together 6F - 3Fk less per step than the
-        # five-pass budget B_alg = 42F + 48Ft + 6Fk, which stays the reported figure
+        # a - z2 N(u) instead of the state: together with the above 6F + 4Ft - 3Fk
+        # less per step than the five-pass budget B_alg = 42F + 48Ft + 6Fk, which
+        # stays the reported figure
         "colE1": 12 * F + 2 * Ft + 3 * Fk,
         "colE2": 12 * F + 2 * Ft,
     }, 42 * F + 48 * Ft + 6 * Fk

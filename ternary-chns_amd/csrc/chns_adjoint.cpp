@@ -178,7 +178,7 @@ int vjp(chns_ctx* c, chns::cd* const* x, int stage, int slot) {
     const chns::Bufs b = bufs_at(c, x, nullptr);
     ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
     ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
-    ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+    ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, false, s));
   }
   ALAUNCH(c, chns::launch_adj_rowD(L, A, s));
   ALAUNCH(c, chns::launch_adj_colBC(L, A, s));
@@ -191,6 +191,10 @@ int vjp(chns_ctx* c, chns::cd* const* x, int stage, int slot) {
 // place).  slot >= 0: passes A-C write intermediate slot `slot` and pass E's
 // fused pass A writes slot next_slot (the next stage's input); full = false
 // stops after pass C (the intermediates of a stage whose output is unused).
+// Passes C and D take the vorticity advection in its gradient form here
+// (pass C writes dw/dx and dw/dy, which the reverse pass D reads); the
+// forward sweep's conservative form is the same function of the state, so
+// the replayed trajectory differs from it by round-off only.
 int replay_stage(chns_ctx* c, chns::cd* const* in, chns::cd* const* out, int stage, bool need_colA, int slot,
                  int next_slot, bool full) {
   const int L = c->d.log2n, M = CHNS_MODEL_TERNARY;
@@ -199,9 +203,9 @@ int replay_stage(chns_ctx* c, chns::cd* const* in, chns::cd* const* out, int sta
   if (slot >= 0) point_slot(c, b, slot);
   if (need_colA) ALAUNCH(c, chns::launch_colA(L, b, c->kp, 2, s));
   ALAUNCH(c, chns::launch_rowB(L, M, b, c->kp, s));
-  ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_colC(L, M, b, c->kp, false, s));
   if (!full) return 0;
-  ALAUNCH(c, chns::launch_rowD(L, M, b, c->kp, s));
+  ALAUNCH(c, chns::launch_rowD(L, M, b, c->kp, false, s));
   if (next_slot >= 0) {
     chns::Bufs nb = c->b;
     point_slot(c, nb, next_slot);

@@ -252,7 +252,8 @@ int chns_create_on(chns_ctx** out, const chns_params* p, int device, hipStream_t
   }
   if ((rc = dalloc(c, &b.flag, 1))) return bail(rc);
   if (p->gravity && (rc = dalloc(c, &b.GR, S))) return bail(rc);
-  if (p->sponge && (rc = dalloc(c, &b.Hw, RJ))) return bail(rc);
+  // T(w) for the sponge, and for pass D's conservative vorticity advection
+  if ((p->sponge || c->nph > 0) && (rc = dalloc(c, &b.Hw, RJ))) return bail(rc);
   b.tw = dtw;
   b.tabw1 = dw1;
   b.tabc1 = dc1;
@@ -367,16 +368,19 @@ int enqueue_step(chns_ctx* c) {
   const chns::KParams& K = c->kp;
   // SMOOTH_FILTER: the passes of chns_filter.hip
   const bool F = B.filt != nullptr;
+  // passes C and D take the vorticity advection in conservative form, except
+  // with the sponge, whose term has no derivative to share
+  const bool cons = !K.sponge;
   for (int st = 1; st <= 2; ++st) {
     if (c->nph > 0)
       CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWB, [&] {
         return F ? chns::launch_filt_rowB(L, M, B, K, s) : chns::launch_rowB(L, M, B, K, s);
       }));
     CHNS_LAUNCH(c, timed(c, CHNS_PASS_COLC, [&] {
-      return F ? chns::launch_filt_colC(L, M, B, K, s) : chns::launch_colC(L, M, B, K, s);
+      return F ? chns::launch_filt_colC(L, M, B, K, s) : chns::launch_colC(L, M, B, K, cons, s);
     }));
     CHNS_LAUNCH(c, timed(c, CHNS_PASS_ROWD, [&] {
-      return F ? chns::launch_filt_rowD(L, M, B, K, s) : chns::launch_rowD(L, M, B, K, s);
+      return F ? chns::launch_filt_rowD(L, M, B, K, s) : chns::launch_rowD(L, M, B, K, cons, s);
     }));
     if (K.grav) CHNS_LAUNCH(c, chns::launch_gravity(B.C1, c->nph == 2 ? B.C2 : nullptr, B.GR, K.gr1, K.gr2, K.f,
                                                     c->d.N, s));

@@ -14,10 +14,10 @@ int launch_batch_rowB_half(int log2n, int model, const Bufs* B, const KParams* P
   CHNS_DISPATCH(log2n, (passes::rowB<L, false, 1>(model, InstBatch{B, P}, nb, s)));
 }
 int launch_batch_rowD_half(int log2n, int model, const Bufs* B, const KParams* P, int nb, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (passes::rowD<L, false, 1>(model, InstBatch{B, P}, nb, s)));
+  CHNS_DISPATCH(log2n, (passes::rowD<L, false, 1>(model, InstBatch{B, P}, nb, s, true)));
 }
 int launch_batch_colC_r16(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (passes::colC<L, false, 16>(model, InstBatch{B, P}, Jt, nb, s)));
+  CHNS_DISPATCH(log2n, (passes::colC<L, false, 16>(model, InstBatch{B, P}, Jt, nb, s, true)));
 }
 int launch_batch_colE_r16(int log2n, int model, int stage, const Bufs* B, const KParams* P, int nf, int nb,
                           hipStream_t s) {

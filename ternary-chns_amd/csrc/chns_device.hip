@@ -32,11 +32,11 @@ int launch_colA(int log2n, const Bufs& B, const KParams& P, int nph, hipStream_t
 int launch_rowB(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
   CHNS_DISPATCH(log2n, (passes::rowB<L>(model, Inst{B, P}, 1, s)));
 }
-int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (passes::colC<L>(model, Inst{B, P}, P.Jt, 1, s)));
+int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, bool cons, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::colC<L>(model, Inst{B, P}, P.Jt, 1, s, cons)));
 }
-int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s) {
-  CHNS_DISPATCH(log2n, (passes::rowD<L>(model, Inst{B, P}, 1, s)));
+int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, bool cons, hipStream_t s) {
+  CHNS_DISPATCH(log2n, (passes::rowD<L>(model, Inst{B, P}, 1, s, cons)));
 }
 int launch_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nf, hipStream_t s) {
   CHNS_DISPATCH(log2n, (passes::colE<L>(model, stage, Inst{B, P}, nf, 1, s)));

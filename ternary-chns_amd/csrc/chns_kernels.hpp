@@ -416,10 +416,13 @@ __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT)) void k_rowB(Args args) {
 
 // ============================================================ pass C
 // grid: 1-D over (tile, group) items (col_item): ternary 6 groups, binary 4, NS 2.
-// Groups are dispatched in order, the one-transform groups (d/dx c_i) last,
-// so they fill the final round of workgroups.
-template <int LOG2N, int MODEL, class Args, bool FILT = false, int PP = 8>
+// Groups are dispatched in order, the one-transform groups (d/dx c_i, and
+// T(w) with CONS) last, so they fill the final round of workgroups.
+// CONS: pass D takes the vorticity advection in conservative form, so the
+// vorticity group writes T(w) (Hw) instead of T(i kx w), T(i ky w).
+template <int LOG2N, int MODEL, class Args, bool FILT = false, int PP = 8, bool CONS = false>
 __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
+  static_assert(!CONS || (!FILT && MODEL != MODEL_NS), "conservative form: pruned binary/ternary passes only");
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
   using CF = ColCfg<LOG2N, PP>;
@@ -474,6 +477,12 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
     fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hv, P.ldJ, j, ok, t, v);
+  } else if (CONS && g == 3) {
+    // T(w) (nonlin.c:36-37 without the derivatives, which pass D's
+    // conservative form takes in the products' transforms)
+    col_load_trunc<LOG2N, CF::P, FILT>(B.W, B.filt, j, a, t, v);
+    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    col_store_rm<LOG2N, CF::P, TJ>(B.Hw, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
     col_load_trunc<LOG2N, CF::P, FILT>(B.W, B.filt, j, a, t, s);
@@ -504,8 +513,12 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
 
 // ============================================================ pass D
 // grid: N / R
-template <int LOG2N, int MODEL, class Args, bool FILT = false, int RV = 0>
+// CONS (binary and ternary with the pruned N/4 cut, no sponge): the
+// vorticity advection in conservative form (see the CONS branch); pass C
+// then writes Hw instead of Hwx, Hwy.
+template <int LOG2N, int MODEL, class Args, bool FILT = false, int RV = 0, bool CONS = false>
 __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT), 2) void k_rowD(Args args) {
+  static_assert(!CONS || (!FILT && MODEL != MODEL_NS), "conservative form: pruned binary/ternary passes only");
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N, RV>;
@@ -558,6 +571,171 @@ __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT), 2) void k_rowD(Args args) 
       for (int m = 0; m < 8; ++m) v[m] = mk(u[m] * (v[m].x * sc) + w[m] * (v[m].y * sc), 0.0);
       cd* o = B.G1 + fo;
       rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o[TF * n] = mk(-A.x, -A.y); });
+    }
+  } else if constexpr (CONS) {
+    // For a divergence-free u = (u, v), u.grad w = d/dx(u w) + d/dy(v w), so
+    // the vorticity right-hand side (nonlin.c:56-92 with the curl of the
+    // surface-tension force, curl.c:16-31)
+    //   -F(u.grad w) - i ky F(Fx) + i kx F(Fy) = i kx F(P) - i ky F(Q),
+    //   P = Fy - u w,  Q = Fx + v w,
+    // takes the row c2r of w itself instead of those of dw/dx and dw/dy, and
+    // one paired forward transform (P, Q) instead of -u.grad w, Fx and Fy:
+    // G1 = -i ky F(Q) (the ky factor of this row transform), G2 = F(P), and
+    // pass E adds i kx F(G2) as before.  Both forms are the same function of
+    // the dealiased state: every kept mode has |kx|, |ky| <= N/4, so the
+    // grid products alias only onto kx = N/2 (from the modes (+-N/4, 0),
+    // where u = 0 and d/dy = 0) and ky = N/2 (from (0, +-N/4), where v = 0
+    // and d/dx = 0), and both forms vanish there; they differ by round-off.
+    // Ternary (binary: mu and w share the first pair, c2 is absent):
+    //   (mu1, mu2) -> A_i; (u, v) -> Q = -u chi/eta, P = -v chi/eta (walls);
+    //   grad c1, grad c2 -> Q += A_i dc_i/dx, P += A_i dc_i/dy, adv_i;
+    //   forward (adv1, adv2); w -> Q += v w, P -= u w; forward (P, Q).
+    // Walls: A_i and adv_i carry the factor 1 - chi (surface_tension.c:
+    // 83-85,151-154,202-205; nonlin.c:209-211,294-296,337-339).
+    constexpr bool TER = MODEL == MODEL_TERNARY;
+    constexpr bool STG = RC::R == 1;   // one row per workgroup: the field pairs are staged in LDS
+    __shared__ cd stgA[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+    __shared__ cd stgB[STG ? 2 * RowStager<LOG2N>::JMAX : 1];
+    const RowStager<LOG2N> sa{stgA, P.Jt}, sb{stgB, P.Jt};
+    const double om = P.vpm ? 1.0 - B.chi[x] : 1.0;
+    const double ce = P.vpm ? B.chi[x] * P.etainv : 0.0;
+    const cd* const hm2 = TER ? B.Hmu2 + ro : B.Hw + ro;
+    double ux[8], uy[8], pp[8], qq[8], ad1[8], ad2[8];
+    {
+      double A1[8], A2[8];   // binary: A2 holds w
+      if constexpr (STG) {
+        sa.template issue<true>(B.Hmu1 + ro, hm2);
+        sb.template issue<true>(B.Hu + ro, B.Hv + ro);
+        sa.template wait_older<true>();
+        sa.template assemble<false, true>(P.f, t, v);
+        lds_sync();
+        sa.template issue<true>(B.Hdx1 + ro, B.Hc1 + ro);
+      } else {
+        row_load_pair<LOG2N, false>(B.Hmu1 + ro, hm2, P.Jt, P.f, t, v);
+      }
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const double m1 = v[m].x * sc, m2 = v[m].y * sc;
+        if (TER) {
+          A1[m] = (P.A11 * m1 + P.A12 * m2) * om;   // surface_tension.c:144
+          A2[m] = (P.A21 * m1 + P.A22 * m2) * om;   // surface_tension.c:195
+        } else {
+          A1[m] = m1 * om;                          // binary mu (surface_tension.c:183-184)
+          A2[m] = m2;                               // w
+        }
+      }
+      if constexpr (STG) {
+        sb.template wait_older<true>();
+        sb.template assemble<false, true>(P.f, t, v);
+        if (TER) {
+          lds_sync();
+          sb.template issue<true>(B.Hdx2 + ro, B.Hc2 + ro);
+        }
+      } else {
+        row_load_pair<LOG2N, false>(B.Hu + ro, B.Hv + ro, P.Jt, P.f, t, v);
+      }
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        ux[m] = v[m].x * sc;
+        uy[m] = v[m].y * sc;
+        if (TER) {
+          qq[m] = -(ux[m] * ce);
+          pp[m] = -(uy[m] * ce);
+        } else {
+          qq[m] = uy[m] * A2[m] - ux[m] * ce;
+          pp[m] = -(ux[m] * A2[m]) - uy[m] * ce;
+        }
+      }
+      if constexpr (STG) {
+        if (TER) {
+          sa.template wait_older<true>();
+        } else {
+          sa.wait();
+        }
+        sa.template assemble<true, true>(P.f, t, v);
+        if (TER) {
+          lds_sync();
+          sa.template issue<false>(B.Hw + ro, nullptr);
+        }
+      } else {
+        row_load_pair<LOG2N, true>(B.Hdx1 + ro, B.Hc1 + ro, P.Jt, P.f, t, v);
+      }
+      rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const double dx = v[m].x * sc, dy = v[m].y * sc;
+        qq[m] += dx * A1[m];
+        pp[m] += dy * A1[m];
+        ad1[m] = dx * ux[m] + dy * uy[m];
+      }
+      if constexpr (TER) {
+        if constexpr (STG) {
+          sb.template wait_older<false>();
+          sb.template assemble<true, true>(P.f, t, v);
+        } else {
+          row_load_pair<LOG2N, true>(B.Hdx2 + ro, B.Hc2 + ro, P.Jt, P.f, t, v);
+        }
+        rfft<LOG2N, true, RC::R, true, !STG>(v, lds, tws, t);   // after a stager barrier
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const double dx = v[m].x * sc, dy = v[m].y * sc;
+          qq[m] += dx * A2[m];
+          pp[m] += dy * A2[m];
+          ad2[m] = dx * ux[m] + dy * uy[m];
+        }
+      }
+    }
+    if (P.vpm) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        ad1[m] *= om;
+        if (TER) ad2[m] *= om;
+      }
+    }
+    cd* o1 = B.G1 + fo;
+    cd* o2 = B.G2 + fo;
+    cd* o3 = B.G3 + fo;
+    const auto put12 = [&](int n, cd A, cd Bq) {
+      const cd iky = ctimes_ik(Bq, P.f * (double)n);
+      o1[TF * n] = mk(-iky.x, -iky.y);
+      o2[TF * n] = A;
+    };
+    if constexpr (TER) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(ad1[m], ad2[m]);
+      cd* o4 = B.G4 + fo;
+      const auto put34 = [&](int n, cd A, cd Bv) {
+        o3[TF * n] = A;
+        o4[TF * n] = Bv;
+      };
+      if constexpr (STG) {
+        // wait for the w copy before the stores of this transform are
+        // issued, so that the wait (vmcnt(0)) is not also for them
+        sa.wait();
+        rfft_split<LOG2N, RC::R, false>(v, lds, tws, t, put34);   // after the wait's barrier
+        sa.template assemble<false, false>(P.f, t, v);
+      } else {
+        rfft_split<LOG2N, RC::R>(v, lds, tws, t, put34);
+        row_load_pair<LOG2N, false>(B.Hw + ro, nullptr, P.Jt, P.f, t, v);
+      }
+      rfft<LOG2N, true, RC::R, true>(v, lds, tws, t);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const double w = v[m].x * sc;
+        v[m] = mk(pp[m] - ux[m] * w, qq[m] + uy[m] * w);
+      }
+      // no other wave touches this wave's lds quarter after the w inverse
+      // transform's exchange
+      rfft_split<LOG2N, RC::R, !STG>(v, lds, tws, t, put12);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(pp[m], qq[m]);
+      rfft_split<LOG2N, RC::R, !STG>(v, lds, tws, t, put12);   // after the grad c inverse transform
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = mk(ad1[m], 0.0);
+      rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd A, cd) { o3[TF * n] = A; });
     }
   } else {
     // Order: mu -> A_i, then velocity, then grad c_i (surface-tension force

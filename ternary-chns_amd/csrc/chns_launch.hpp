@@ -10,8 +10,10 @@ struct cd;
 
 int launch_colA(int log2n, const Bufs& B, const KParams& P, int nphase, hipStream_t s);
 int launch_rowB(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
-int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
-int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, hipStream_t s);
+// cons: the conservative vorticity advection (pass C writes Hw for pass D
+// instead of Hwx, Hwy); the same value for passes C and D of a stage
+int launch_colC(int log2n, int model, const Bufs& B, const KParams& P, bool cons, hipStream_t s);
+int launch_rowD(int log2n, int model, const Bufs& B, const KParams& P, bool cons, hipStream_t s);
 int launch_colE(int log2n, int model, int stage, const Bufs& B, const KParams& P, int nfields, hipStream_t s);
 int launch_row_r2c(int log2n, const double* a, const double* b, cd* oa, cd* ob, const cd* tw, int ldF,
                    hipStream_t s);
@@ -34,7 +36,8 @@ int launch_filt_colE(int log2n, int model, int stage, const Bufs& B, const KPara
 
 // batched passes (chns_batch.hip): nb instances of one size and model, device
 // arrays B[nb], P[nb]; sched: the group's FFT schedule (bit 0: radix-16
-// column passes, bit 1: half the rows per row workgroup)
+// column passes, bit 1: half the rows per row workgroup).  Passes C and D
+// take the conservative vorticity advection (ensembles have no sponge).
 int launch_batch_colA(int log2n, const Bufs* B, const KParams* P, int Jt, int nph, int nb, hipStream_t s);
 int launch_batch_rowB(int log2n, int model, const Bufs* B, const KParams* P, int nb, int sched, hipStream_t s);
 int launch_batch_colC(int log2n, int model, const Bufs* B, const KParams* P, int Jt, int nb, int sched,

@@ -25,20 +25,39 @@ template <int L, bool FILT = false, int RV = 0, class A> int rowB(int model, con
   else k_rowB<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, int PP = 8, class A> int colC(int model, const A& a, int Jt, int nb, hipStream_t s) {
+// cons: the conservative vorticity advection of passes C and D (binary and
+// ternary with the pruned cut; NS and the unpruned passes ignore it).  Passes
+// C and D of one stage must agree on it.
+template <int L, bool FILT = false, int PP = 8, class A>
+int colC(int model, const A& a, int Jt, int nb, hipStream_t s, bool cons = false) {
   const dim3 g(round16(ceil_div<L, PP>(Jt) * groupsC(model)), 1, nb);
   constexpr int NT = ColCfg<L, PP>::NT;
-  if (model == MODEL_TERNARY) k_colC<L, MODEL_TERNARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_colC<L, MODEL_BINARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
-  else k_colC<L, MODEL_NS, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  constexpr bool CF = !FILT;   // the conservative form has pruned instantiations only
+  if (model == MODEL_TERNARY) {
+    if (CF && cons) k_colC<L, MODEL_TERNARY, A, FILT, PP, CF><<<g, NT, 0, s>>>(a);
+    else k_colC<L, MODEL_TERNARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  } else if (model == MODEL_BINARY) {
+    if (CF && cons) k_colC<L, MODEL_BINARY, A, FILT, PP, CF><<<g, NT, 0, s>>>(a);
+    else k_colC<L, MODEL_BINARY, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  } else {
+    k_colC<L, MODEL_NS, A, FILT, PP><<<g, NT, 0, s>>>(a);
+  }
   return (int)hipGetLastError();
 }
-template <int L, bool FILT = false, int RV = 0, class A> int rowD(int model, const A& a, int nb, hipStream_t s) {
+template <int L, bool FILT = false, int RV = 0, class A>
+int rowD(int model, const A& a, int nb, hipStream_t s, bool cons = false) {
   using RC = RowCfg<L, RV>;
   const dim3 g((1 << L) / RC::R, 1, nb);
-  if (model == MODEL_TERNARY) k_rowD<L, MODEL_TERNARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
-  else if (model == MODEL_BINARY) k_rowD<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
-  else k_rowD<L, MODEL_NS, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  constexpr bool CF = !FILT;
+  if (model == MODEL_TERNARY) {
+    if (CF && cons) k_rowD<L, MODEL_TERNARY, A, FILT, RV, CF><<<g, RC::NT, 0, s>>>(a);
+    else k_rowD<L, MODEL_TERNARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  } else if (model == MODEL_BINARY) {
+    if (CF && cons) k_rowD<L, MODEL_BINARY, A, FILT, RV, CF><<<g, RC::NT, 0, s>>>(a);
+    else k_rowD<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  } else {
+    k_rowD<L, MODEL_NS, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
+  }
   return (int)hipGetLastError();
 }
 template <int L, int ST, bool FILT = false, int PP = 8, class A>

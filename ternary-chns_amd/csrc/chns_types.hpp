@@ -38,7 +38,8 @@ struct Bufs {
   cd *Hc1, *Hc2;               // row-major [N][ldJ]
   cd *Hu, *Hv, *Hwx, *Hwy, *Hmu1, *Hmu2, *Hdx1, *Hdx2;
   cd *Mh1, *Mh2;               // row-major [N][ldF]
-  cd *G1, *G2, *G3, *G4;       // W1, Fy, adv1, adv2 (row-major [N][ldF])
+  cd *G1, *G2, *G3, *G4;       // W1, Fy, adv1, adv2 (row-major [N][ldF]); conservative pass D:
+                               // -i ky F(Fx + v w), F(Fy - u w), adv1, adv2
   const cd* tw;                // twiddles exp(-2 pi i n/N)
   // ETD2RK coefficients, [(N/2+1)*(N/2+1)] indexed j*(N/2+1) + |i|, one
   // table per field kind so each field group reads only its own
@@ -53,7 +54,7 @@ struct Bufs {
   const double* chi;           // [N] wall profile chi(x) (vpm)
   const double* spg;           // [N] sponge(y)/eta (sponge)
   cd* GR;                      // buoyancy torque, column-major (gravity)
-  cd* Hw;                      // column iFFT of T(omega), TJ tiled (sponge)
+  cd* Hw;                      // column iFFT of T(omega), TJ tiled (sponge; conservative pass D)
   int* flag;                   // divergence flag: set to 1 by pass E when an updated mode is not finite
 };
 
