@@ -66,7 +66,9 @@ def kernel_tag(pass_name: str, N: int) -> str:
     # ensembles use chns::InstBatch
     row, col = "chns::Inst, false, 0>", "chns::Inst, false, 8>"
     rowc, colc = "chns::Inst, false, 0, true>", "chns::Inst, false, 8, true>"
-    return {"colA": f"k_colA<{L}, chns::Inst, false>", "rowB": f"k_rowB<{L}, 2, {row}",
+    # pass B at 2048 is the row-pipelined kernel
+    rowb = f"k_rowB_pipe<{L}, 2, chns::Inst>" if L == 11 else f"k_rowB<{L}, 2, {row}"
+    return {"colA": f"k_colA<{L}, chns::Inst, false>", "rowB": rowb,
             "colC": f"k_colC<{L}, 2, {colc}", "rowD": f"k_rowD<{L}, 2, {rowc}",
             "colE1": f"k_colE<{L}, 2, 1, {col}", "colE2": f"k_colE<{L}, 2, 2, {col}"}[pass_name]
 

@@ -88,15 +88,18 @@ __device__ __forceinline__ size_t tix(int x, int j, int ld) {
 // b, b+8, b+16, ... which sit on the same XCD (blocks are dealt to the 8 XCDs
 // round-robin) and start together: their partial-line writes of the outputs
 // merge in that L2, and the two rows of an input pair read one line.
-template <int LOG2N, int RV = 0> __device__ __forceinline__ int row_of(int r) {
+template <int LOG2N, int RV = 0> __device__ __forceinline__ int row_of_block(int b, int r) {
   constexpr int R = RowCfg<LOG2N, RV>::R;
   if constexpr (R >= TF) {
-    return blockIdx.x * R + r;
+    return b * R + r;
   } else {
     constexpr int G = TF / R;   // workgroups per output tile
-    const int b = blockIdx.x, xcd = b & 7, idx = b >> 3;
+    const int xcd = b & 7, idx = b >> 3;
     return TF * (((idx / G) << 3) + xcd) + (idx % G) * R + r;
   }
+}
+template <int LOG2N, int RV = 0> __device__ __forceinline__ int row_of(int r) {
+  return row_of_block<LOG2N, RV>(blockIdx.x, r);
 }
 
 // Kernel arguments of the pass kernels: one instance passed by value, or a
@@ -366,13 +369,34 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
 }
 
 // ============================================================ pass B
+// The pointwise part of pass B at one point: muF_1, muF_2 times fac
+// (chem_potential.c:40-86) for the ternary model, P(c) = c(1-c)(1-2c)
+// (surface_tension.c:126-127, nonlin.c:225) for the binary one.
+template <int MODEL>
+__device__ __forceinline__ cd muf_point(double c1, double c2, const KParams& P) {
+  if constexpr (MODEL == MODEL_TERNARY) {
+    const double c3 = (1.0 - c1 - c2);
+    const double c123 = c1 * c2 * c3;
+    const double fc1 = c1 * (1.0 - c1) * (1.0 - 2.0 * c1);
+    const double fc2 = c2 * (1.0 - c2) * (1.0 - 2.0 * c2);
+    const double fc3 = c3 * (1.0 - c3) * (1.0 - 2.0 * c3);
+    const double G1 = P.g1 * fc1 + P.X6 * c123 * c2 * c3;
+    const double G2 = P.g2 * fc2 + P.X6 * c123 * c1 * c3;
+    const double G3 = P.g3 * fc3 + P.X6 * c123 * c1 * c2;
+    const double m1 = (G1 - G2) * P.g2i + (G1 - G3) * P.g3i;
+    const double m2 = (G2 - G1) * P.g1i + (G2 - G3) * P.g3i;
+    return mk(m1 * P.fac, m2 * P.fac);
+  } else {
+    return mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
+  }
+}
+
 // grid: N / R
 template <int LOG2N, int MODEL, class Args, bool FILT = false, int RV = 0>
 __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT)) void k_rowB(Args args) {
   const Bufs& B = inst_bufs(args);
   const KParams& P = inst_kp(args);
   using RC = RowCfg<LOG2N, RV>;
-  constexpr int N = RC::N, T = RC::T;
   __shared__ cd lds_all[RowLds<LOG2N, RC::R>::value];
   const int r = threadIdx.x % RC::R, t = threadIdx.x / RC::R;   // rows interleaved across lanes
   const int x = row_of<LOG2N, RV>(r);
@@ -385,33 +409,96 @@ __global__ __launch_bounds__((RowCfg<LOG2N, RV>::NT)) void k_rowB(Args args) {
   row_load_pair<LOG2N, false>(h1, h2, P.Jt, P.f, t, v);
   rfft<LOG2N, true, RC::R, !FILT>(v, lds, tws, t);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) {
-    const double c1 = v[m].x * P.scale;
-    if constexpr (MODEL == MODEL_TERNARY) {
-      // chem_potential.c:40-86
-      const double c2 = v[m].y * P.scale;
-      const double c3 = (1.0 - c1 - c2);
-      const double c123 = c1 * c2 * c3;
-      const double fc1 = c1 * (1.0 - c1) * (1.0 - 2.0 * c1);
-      const double fc2 = c2 * (1.0 - c2) * (1.0 - 2.0 * c2);
-      const double fc3 = c3 * (1.0 - c3) * (1.0 - 2.0 * c3);
-      const double G1 = P.g1 * fc1 + P.X6 * c123 * c2 * c3;
-      const double G2 = P.g2 * fc2 + P.X6 * c123 * c1 * c3;
-      const double G3 = P.g3 * fc3 + P.X6 * c123 * c1 * c2;
-      const double m1 = (G1 - G2) * P.g2i + (G1 - G3) * P.g3i;
-      const double m2 = (G2 - G1) * P.g1i + (G2 - G3) * P.g3i;
-      v[m] = mk(m1 * P.fac, m2 * P.fac);
-    } else {
-      // P(c) = c(1-c)(1-2c)  (surface_tension.c:126-127, nonlin.c:225)
-      v[m] = mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
-    }
-  }
+  for (int m = 0; m < 8; ++m) v[m] = muf_point<MODEL>(v[m].x * P.scale, v[m].y * P.scale, P);
   cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
   cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
   rfft_split<LOG2N, RC::R>(v, lds, tws, t, [&](int n, cd a, cd b) {
     o1[TF * n] = a;
     if (MODEL == MODEL_TERNARY) o2[TF * n] = b;
   });
+}
+
+// Pass B at N = 2048 with the pruned N/4 cut, one row per workgroup at a
+// time: workgroup b takes the rows of logical blocks b, b + G, b + 2G, ...
+// (G = gridDim.x, a multiple of 32 so the rows of one output tile still run
+// together on one XCD) and issues the next row's loads before transforming
+// the current one, so they land while it computes instead of every row
+// waiting for its own (a grid of one row per workgroup runs in rounds whose
+// loads, transforms and stores do not overlap).  At 3 waves per SIMD (the
+// next row's entries take 32 registers) G = 768 fills every CU once.
+// With Jt <= N/4 + 1 a thread's nonzero row entries are those of its points
+// m = 0, 1, 6, 7 (k = t, 256 + t, 512 - t, 256 - t), and thread 0's point
+// m = 2 (n = 512), whose entry k = 512 is that of its point m = 6.
+constexpr int kRowPipeM[4] = {0, 1, 6, 7};
+struct RowRaw {
+  cd a[4], b[4];
+};
+template <int LOG2N>
+__device__ __forceinline__ void row_raw_load(const cd* __restrict__ X, const cd* __restrict__ Y, int Jt, int t,
+                                             RowRaw& r) {
+  constexpr int N = 1 << LOG2N, T = N / 8;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int n = t + kRowPipeM[q] * T;
+    const int k = n <= N / 2 ? n : N - n;
+    r.a[q] = k < Jt ? X[TJ * k] : mk(0.0, 0.0);
+    r.b[q] = (Y && k < Jt) ? Y[TJ * k] : mk(0.0, 0.0);
+  }
+}
+// Z = X + iY at the thread's points (assemble_pair on the raw entries)
+template <int LOG2N>
+__device__ __forceinline__ void row_raw_assemble(const RowRaw& r, int t, cd* v) {
+  constexpr int N = 1 << LOG2N, T = N / 8;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = mk(0.0, 0.0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int m = kRowPipeM[q];
+    const int n = t + m * T;
+    const cd a = r.a[q], b = r.b[q];
+    if (n == 0) v[m] = mk(a.x, b.x);
+    else if (n < N / 2) v[m] = mk(a.x - b.y, a.y + b.x);
+    else v[m] = mk(a.x + b.y, b.x - a.y);
+  }
+  if (t == 0) v[2] = mk(r.a[2].x - r.b[2].y, r.a[2].y + r.b[2].x);   // n = 512 < N/2
+}
+template <int LOG2N, int MODEL, class Args>
+__global__ __launch_bounds__(256, 3) void k_rowB_pipe(Args args) {
+  static_assert(LOG2N == 11, "one row per 256-thread workgroup");
+  const Bufs& B = inst_bufs(args);
+  const KParams& P = inst_kp(args);
+  constexpr int N = 1 << LOG2N;
+  __shared__ cd lds[RowLds<LOG2N, 1>::value];
+  const int t = threadIdx.x;
+  constexpr bool TER = MODEL == MODEL_TERNARY;
+  RowTw<LOG2N, 1> tws;
+  tws.init(B.tw, t);
+  int L = blockIdx.x;
+  const int G = gridDim.x;
+  RowRaw raw;
+  {
+    const int x = row_of_block<LOG2N, 0>(L, 0);
+    row_raw_load<LOG2N>(B.Hc1 + tix<TJ>(x, 0, P.ldJ), TER ? B.Hc2 + tix<TJ>(x, 0, P.ldJ) : nullptr, P.Jt, t, raw);
+  }
+  for (; L < N; L += G) {
+    const int x = row_of_block<LOG2N, 0>(L, 0);
+    cd v[8];
+    row_raw_assemble<LOG2N>(raw, t, v);
+    if (L + G < N) {
+      const int xn = row_of_block<LOG2N, 0>(L + G, 0);
+      row_raw_load<LOG2N>(B.Hc1 + tix<TJ>(xn, 0, P.ldJ), TER ? B.Hc2 + tix<TJ>(xn, 0, P.ldJ) : nullptr, P.Jt, t,
+                          raw);
+    }
+    rfft<LOG2N, true, 1, true>(v, lds, tws, t);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = muf_point<MODEL>(v[m].x * P.scale, v[m].y * P.scale, P);
+    cd* o1 = B.Mh1 + tix<TF>(x, 0, P.ldF);
+    cd* o2 = B.Mh2 + tix<TF>(x, 0, P.ldF);
+    rfft_split<LOG2N, 1>(v, lds, tws, t, [&](int n, cd a, cd b) {
+      o1[TF * n] = a;
+      if (TER) o2[TF * n] = b;
+    });
+  }
 }
 
 // ============================================================ pass C

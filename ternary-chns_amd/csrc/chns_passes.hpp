@@ -20,6 +20,14 @@ template <int L, bool FILT = false, class A> int colA(const A& a, int Jt, int np
 }
 template <int L, bool FILT = false, int RV = 0, class A> int rowB(int model, const A& a, int nb, hipStream_t s) {
   using RC = RowCfg<L, RV>;
+  if constexpr (L == 11 && RV == 0 && !FILT) {
+    // 768 workgroups (3 per CU: 140 VGPRs) taking 2 or 3 rows in turn, each
+    // row's loads under the previous row's transforms (k_rowB_pipe)
+    const dim3 g((1 << L) / 8 * 3, 1, nb);
+    if (model == MODEL_TERNARY) k_rowB_pipe<L, MODEL_TERNARY, A><<<g, RC::NT, 0, s>>>(a);
+    else k_rowB_pipe<L, MODEL_BINARY, A><<<g, RC::NT, 0, s>>>(a);
+    return (int)hipGetLastError();
+  }
   const dim3 g((1 << L) / RC::R, 1, nb);
   if (model == MODEL_TERNARY) k_rowB<L, MODEL_TERNARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
   else k_rowB<L, MODEL_BINARY, A, FILT, RV><<<g, RC::NT, 0, s>>>(a);
