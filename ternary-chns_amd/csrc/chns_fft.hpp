@@ -195,8 +195,14 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS
       for (int q = 0; q < R; ++q) v[u + q * Q] = a[q];
     } else {
       const int base = (b / NS) * NS * R + k;
+      // lp(base + q NS) = lp(base) + q NS + (q NS)/8: NS is 1 (then base = b R,
+      // a multiple of 8) or a multiple of 8, so adding q NS carries its own
+      // eighths; the writes then take constant offsets from one address
+      static_assert(NS == 1 || NS % 8 == 0, "stage strides");
+      static_assert(CHNS_LDS_PAD == 3, "offsets assume one pad per 8 entries");
+      const int lb = lp(base);
 #pragma unroll
-      for (int q = 0; q < R; ++q) lds[lp(base + q * NS) * S] = a[q];
+      for (int q = 0; q < R; ++q) lds[(lb + q * NS + ((q * NS) >> 3)) * S] = a[q];
     }
   }
 }
@@ -217,8 +223,15 @@ __device__ __forceinline__ void lds_sync() {
 template <int LOG2N, int P, int S>
 __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
   constexpr int T = (1 << LOG2N) / P;
+  if constexpr (T % 8 == 0) {
+    // lp(t + m T) = lp(t) + m (T + T/8): constant offsets from one address
+    const int lt = lp(t);
 #pragma unroll
-  for (int m = 0; m < P; ++m) v[m] = lds[lp(t + m * T) * S];
+    for (int m = 0; m < P; ++m) v[m] = lds[(lt + m * (T + T / 8)) * S];
+  } else {
+#pragma unroll
+    for (int m = 0; m < P; ++m) v[m] = lds[lp(t + m * T) * S];
+  }
 }
 
 template <int LOG2N, int P, bool INV, int S, int NS, bool PRUNED>
