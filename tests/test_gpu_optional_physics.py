@@ -92,6 +92,28 @@ def test_full_size_sponge_one_step():
         assert O.rel_l2(a - a0, b - a0) < 1e-10
 
 
+@pytest.mark.parametrize("model", ["binary", "ternary"])
+def test_full_size_walls_one_step(model):
+    """2048^2 with the walls and no sponge: pass D's one-row-per-workgroup path with the
+    conservative vorticity advection, where the walls scale A_i and u.grad c_i by 1 - chi
+    and put -u chi/eta, -v chi/eta into Q and P (the 32^2/64^2 cases run the same form
+    through the several-rows-per-workgroup path)."""
+    N = 2048
+    op, cp, _ = case(model, N, vpm=1, nu=2e-3, lam=5e-4, dt=1e-6)
+    if model == "ternary":
+        reals = (O.random_vorticity(op, 2),) + tuple(O.init_lens_real(op))
+    else:
+        reals = (O.random_vorticity(op, 2),) + O.init_noise_real(op, 3, 0.5, 0.1)
+    s0 = O.state_from_real(op, *reals)
+    s1 = O.etd2rk(op, s0)
+    with C.Context(cp) as ctx:
+        ctx.set_state(*s0.fields())
+        ctx.step(1)
+        g = ctx.get_state()
+    for a, b, a0 in zip(g, s1.fields(), s0.fields()):
+        assert O.rel_l2(a - a0, b - a0) < 1e-10
+
+
 def test_switch_rejections():
     kw = dict(Nx=32, Ny=32, lx=TWO_PI, ly=TWO_PI)
     with pytest.raises(C.ChnsError):
