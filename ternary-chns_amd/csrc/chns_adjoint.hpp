@@ -321,12 +321,13 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colBC(AdjArgs args) {
   const bool ok = j < P.Jt;
   const int a = ok ? A.amask[j] : -1;
   cd* lds = lds_all + c;
+  COL_TWIDDLES(A.tw);
   const double ky = P.f * (double)j;
   const double wj = (j == 0) ? 1.0 : 2.0;   // half-spectrum weight (j = N/2 is never kept)
   cd v[CF::P], s[CF::P], gi[CF::P];
   // Mbar_i = T(R(mubar_i)) - (lambda/g_i) k^2 g_i
   col_load_rm<LOG2N, CF::P, TF>(A.O[g == 0 ? 6 : 9], P.ldJ, j, ok, t, v);
-  fft<LOG2N, CF::P, false, CF::C>(v, lds, A.tw, t);
+  fft<LOG2N, CF::P, false, CF::C, false, LT>(v, lds, twp, t);
   col_load_masked<LOG2N, CF::P>(g == 0 ? A.C1 : A.C2, j, a, t, s);
   col_load_masked<LOG2N, CF::P>(A.Gi[1 + g], j, a, t, gi);
   const double lg = g == 0 ? P.lg1 : P.lg2;
@@ -344,14 +345,14 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colBC(AdjArgs args) {
   }
   // lg partial: -<g_i, k^2 M_i> with M_i = the column FFT of pass B's output
   col_load_rm<LOG2N, CF::P, TF>(g == 0 ? A.Mh1 : A.Mh2, P.ldF, j, ok, t, s);
-  fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+  fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
   double slg = 0.0;
 #pragma unroll
   for (int m = 0; m < CF::P; ++m) {
     const double ks = ksq(kxof<LOG2N>(t + m * T, P.f), ky);
     slg -= wj * ks * (gi[m].x * s[m].x + gi[m].y * s[m].y);   // gi is zero off the kept modes
   }
-  fft<LOG2N, CF::P, true, CF::C, true>(v, lds, A.tw, t);
+  fft<LOG2N, CF::P, true, CF::C, true, LT>(v, lds, twp, t);
   col_store_rm<LOG2N, CF::P, TJ>(g == 0 ? A.Hmb1 : A.Hmb2, P.ldJ, j, ok, t, v);
   sq[g] = smc * P.scale;
   sq[2 + g] = slg * P.scale;
@@ -450,15 +451,16 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
   const bool ok = j < P.Jt;
   const int a = ok ? A.amask[j] : -1;
   cd* lds = lds_all + c;
+  COL_TWIDDLES(A.tw);
   const double ky = P.f * (double)j;
   cd v[CF::P], s[CF::P];
   if constexpr (STAGE != 0) {
     if (g == 0) {
       // omega: T[ Q (R(-i ky ubar) + i kx R(vbar)) - i kx R(w_x bar) - i ky R(w_y bar) ]
       col_load_rm<LOG2N, CF::P, TF>(A.O[0], P.ldJ, j, ok, t, v);
-      fft<LOG2N, CF::P, false, CF::C>(v, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(v, lds, twp, t);
       col_load_rm<LOG2N, CF::P, TF>(A.O[1], P.ldJ, j, ok, t, s);
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) {
         const double kx = kxof<LOG2N>(t + m * T, P.f);
@@ -467,11 +469,11 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
         v[m] = ks > 1e-12 ? mk(q.x / ks, q.y / ks) : mk(0.0, 0.0);   // psi = w / k^2 (velocity.c:20-25)
       }
       col_load_rm<LOG2N, CF::P, TF>(A.O[2], P.ldJ, j, ok, t, s);
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f)));
       col_load_rm<LOG2N, CF::P, TF>(A.O[3], P.ldJ, j, ok, t, s);
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], s[m]);
     } else {
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
 #pragma unroll
         for (int m = 0; m < CF::P; ++m) s[m] = cadd(s[m], w[m]);
       }
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
       col_load_rm<LOG2N, CF::P, TF>(A.O[o + 1], P.ldJ, j, ok, t, s);
@@ -497,11 +499,11 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
 #pragma unroll
         for (int m = 0; m < CF::P; ++m) s[m] = cadd(s[m], w[m]);
       }
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], s[m]);
       col_load_rm<LOG2N, CF::P, TF>(A.O[o + 2], P.ldJ, j, ok, t, s);
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, A.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
       const double mc = g == 1 ? P.mc1 : P.mc2;
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) {
@@ -545,12 +547,12 @@ __global__ __launch_bounds__(ColCfg<LOG2N>::NT) void k_adj_colE(AdjArgs args) {
   if (g == 0) {
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, true>(s, lds, A.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true, LT>(s, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(A.Hxi, P.ldJ, j, ok, t, s);
-    fft<LOG2N, CF::P, true, CF::C, true>(v, lds, A.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(A.Hxix, P.ldJ, j, ok, t, v);
   } else {
-    fft<LOG2N, CF::P, true, CF::C, true>(s, lds, A.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, true, LT>(s, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 1 ? A.Het1 : A.Het2, P.ldJ, j, ok, t, s);
   }
 }

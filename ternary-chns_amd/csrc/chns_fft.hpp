@@ -13,8 +13,10 @@
 // the same transforms in the same order (inputs of idle lanes are zero).
 //
 // Twiddles come from a table tw[n] = exp(-2*pi*i*n/N), n < N, built on the
-// host in extended precision; the inverse uses the conjugates.  No
-// normalisation is applied (the 1/(Nx*Ny) of the c2r is applied by callers).
+// host in extended precision, or from a workgroup's LDS table of each
+// stage's base twiddles filled from it (TwLds, the column kernels); the
+// inverse uses the conjugates.  No normalisation is applied (the 1/(Nx*Ny)
+// of the c2r is applied by callers).
 #pragma once
 #include "chns_types.hpp"
 
@@ -112,27 +114,84 @@ template <bool INV> __device__ __forceinline__ cd twid(const cd* __restrict__ tw
 __device__ __forceinline__ int lp(int i) { return i + (i >> CHNS_LDS_PAD); }
 template <int LOG2N> struct LdsSize { static constexpr int value = (1 << LOG2N) + ((1 << LOG2N) >> CHNS_LDS_PAD); };
 
-// Table twiddles of one stage for thread t: w, w^2, w^4 (, w^8) of each of
-// its Q = P/R butterflies, w = exp(-+2 pi i k TWS/N), k = b mod NS.  They are
-// loaded one stage ahead, before the LDS exchange that precedes their use, so
-// the load latency hides behind the exchange.
-template <int LOG2N, int P, int NS>
+// cos and sin of 2 pi e/16, e < 16
+__device__ __forceinline__ constexpr double cos16(int e) {
+  constexpr double c[16] = {1.0, kC8, kR, kS8, 0.0, -kS8, -kR, -kC8, -1.0, -kC8, -kR, -kS8, 0.0, kS8, kR, kC8};
+  return c[e & 15];
+}
+__device__ __forceinline__ constexpr double sin16(int e) { return cos16(e + 12); }
+__device__ __forceinline__ cd csq(cd a) { return mk(a.x * a.x - a.y * a.y, 2.0 * (a.x * a.y)); }
+
+// Entries of the per-workgroup LDS stage-twiddle table of an N-point
+// transform with P points per thread: for each stage after the first, the
+// base twiddles w = exp(-2 pi i k TWS/N) of its butterflies, k < NS for the
+// radix-P stages, k < T for a smaller final radix (its butterflies u > 0 take
+// k + u T = w(k) exp(-2 pi i u/P), a constant rotation).  Higher powers are
+// formed by squaring.  N = 2048, P = 8: 8 + 64 + 256 = 328 entries (5.1 KiB).
+template <int LOG2N, int P>
+struct TwLds {
+  static constexpr int N = 1 << LOG2N, T = N / P;
+  static constexpr int entries(int NS) { return (N / NS) >= P ? NS : T; }
+  static constexpr int off(int NS) {
+    int o = 0;
+    for (int s = P; s < NS; s *= P) o += entries(s);
+    return o;
+  }
+  static constexpr int size = off(N);
+  // fills the table from the global table tw[n] = exp(-2 pi i n/N); threads
+  // tid < size of the workgroup take one entry each
+  static __device__ __forceinline__ cd entry(const cd* __restrict__ tw, int e) {
+    int NS = P, o = 0;
+#pragma unroll
+    for (int s = 0; s < LOG2N; ++s) {
+      if (NS < N && e >= o + entries(NS)) { o += entries(NS); NS *= P; }
+    }
+    const int R = (N / NS) >= P ? P : (N / NS);
+    return tw[(e - o) * (N / (NS * R))];
+  }
+};
+
+// Twiddles of one stage for thread t: w, w^2, w^4 (, w^8) of each of its Q =
+// P/R butterflies, w = exp(-+2 pi i k TWS/N), k = b mod NS.  They are loaded
+// one stage ahead, before the LDS exchange that precedes their use, so the
+// load latency hides behind the exchange.  LT = false: every power from the
+// global table.  LT = true: the base power of butterfly 0 from the LDS table
+// (TwLds), the others formed at use, so the transform issues no global load.
+template <int LOG2N, int P, int NS, bool LT = false>
 struct StageTw {
   static constexpr int N = 1 << LOG2N;
   static constexpr int R = (N / NS) >= P ? P : (N / NS);
   static constexpr int Q = P / R;
   static constexpr int NL = R >= 16 ? 4 : (R >= 8 ? 3 : (R >= 4 ? 2 : 1));
-  cd w[Q][NL];
+  cd w[LT ? 1 : Q][LT ? 1 : NL];
   template <bool INV>
   __device__ __forceinline__ void load(const cd* __restrict__ tw, int t) {
     if constexpr (NS > 1) {
       constexpr int T = N / P, TWS = N / (NS * R);
+      if constexpr (LT) {
+        w[0][0] = tw[TwLds<LOG2N, P>::off(NS) + (Q == 1 ? t % NS : t)];
+      } else {
 #pragma unroll
-      for (int u = 0; u < Q; ++u) {
-        const int k = (t + u * T) % NS;
+        for (int u = 0; u < Q; ++u) {
+          const int k = (t + u * T) % NS;
 #pragma unroll
-        for (int l = 0; l < NL; ++l) w[u][l] = twid<INV>(tw, (k << l) * TWS);
+          for (int l = 0; l < NL; ++l) w[u][l] = twid<INV>(tw, (k << l) * TWS);
+        }
       }
+    }
+  }
+  // powers w^(2^l), l < NL, of butterfly u
+  template <bool INV>
+  __device__ __forceinline__ void powers(int u, cd* p) const {
+    if constexpr (LT) {
+      cd b = INV ? mk(w[0][0].x, -w[0][0].y) : w[0][0];
+      if (u > 0) b = wmul<INV>(b, cos16(u * 16 / P), sin16(u * 16 / P));
+      p[0] = b;
+#pragma unroll
+      for (int l = 1; l < NL; ++l) p[l] = csq(p[l - 1]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < NL; ++l) p[l] = w[u][l];
     }
   }
 };
@@ -140,8 +199,8 @@ struct StageTw {
 // One Stockham stage of radix R on the cyclic registers v[P] of thread t.
 // NS = product of the radices of earlier stages.  LAST: outputs stay in v in
 // cyclic order; otherwise outputs go to lds at their Stockham positions.
-template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S, bool PRUNED>
-__device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS>& tws, int t) {
+template <int LOG2N, int P, int R, int NS, bool INV, bool LAST, int S, bool PRUNED, class Tw>
+__device__ __forceinline__ void stage(cd* v, cd* lds, const Tw& tws, int t) {
   constexpr int N = 1 << LOG2N;
   constexpr int T = N / P;
   constexpr int Q = P / R;            // butterflies per thread
@@ -159,23 +218,25 @@ __device__ __forceinline__ void stage(cd* v, cd* lds, const StageTw<LOG2N, P, NS
       for (int m = P / 4 + 1; m < 3 * P / 4; ++m) a[m] = mk(0.0, 0.0);
     }
     if constexpr (NS > 1) {
-      // w^m for m < R from the loaded w, w^2, w^4, w^8, applied as they are
-      // formed so few powers are live at once
-      const cd w1 = tws.w[u][0];
+      // w^m for m < R from w, w^2, w^4, w^8, applied as they are formed so
+      // few powers are live at once
+      cd pw[Tw::NL];
+      tws.template powers<INV>(u, pw);
+      const cd w1 = pw[0];
       a[1] = cmulc(a[1], w1);
       if constexpr (R >= 4) {
-        const cd w2 = tws.w[u][1];
+        const cd w2 = pw[1];
         const cd w3 = cmulc(w1, w2);
         a[2] = cmulc(a[2], w2);
         a[3] = cmulc(a[3], w3);
         if constexpr (R >= 8) {
-          const cd w4 = tws.w[u][2];
+          const cd w4 = pw[2];
           a[4] = cmulc(a[4], w4);
           a[5] = cmulc(a[5], cmulc(w1, w4));
           a[6] = cmulc(a[6], cmulc(w2, w4));
           a[7] = cmulc(a[7], cmulc(w3, w4));
           if constexpr (R == 16) {
-            const cd w8 = tws.w[u][3];
+            const cd w8 = pw[3];
             const cd w12 = cmulc(w4, w8);
             a[8] = cmulc(a[8], w8);
             a[9] = cmulc(a[9], cmulc(w1, w8));
@@ -234,20 +295,20 @@ __device__ __forceinline__ void lds_read_cyclic(cd* v, const cd* lds, int t) {
   }
 }
 
-template <int LOG2N, int P, bool INV, int S, int NS, bool PRUNED>
+template <int LOG2N, int P, bool INV, int S, int NS, bool PRUNED, bool LT>
 __device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ tw, int t,
-                                         const StageTw<LOG2N, P, NS>& tws) {
+                                         const StageTw<LOG2N, P, NS, LT>& tws) {
   constexpr int N = 1 << LOG2N;
   constexpr int R = (N / NS) >= P ? P : (N / NS);
   constexpr bool LAST = NS * R == N;
   stage<LOG2N, P, R, NS, INV, LAST, S, PRUNED && NS == 1>(v, lds, tws, t);
   if constexpr (!LAST) {
-    StageTw<LOG2N, P, NS * R> nxt;
+    StageTw<LOG2N, P, NS * R, LT> nxt;
     nxt.template load<INV>(tw, t);   // in flight during the exchange
     lds_sync();
     lds_read_cyclic<LOG2N, P, S>(v, lds, t);
     lds_sync();
-    fft_from<LOG2N, P, INV, S, NS * R, false>(v, lds, tw, t, nxt);
+    fft_from<LOG2N, P, INV, S, NS * R, false, LT>(v, lds, tw, t, nxt);
   }
 }
 
@@ -258,7 +319,9 @@ __device__ __forceinline__ void fft_from(cd* v, cd* lds, const cd* __restrict__ 
 // Begins with a barrier, so back-to-back calls sharing lds are safe.
 // PRUNED: the input is a dealiased (truncated) spectrum, zero for indices
 // N/4 < n < 3N/4; the first stage skips those points.
-template <int LOG2N, int P, bool INV, int S, bool PRUNED = false>
+// LT: tw is the workgroup's LDS stage-twiddle table (TwLds, filled before the
+// first call) instead of the global table.
+template <int LOG2N, int P, bool INV, int S, bool PRUNED = false, bool LT = false>
 __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, int t) {
   static_assert(P == 8 || P == 16, "8 or 16 points per thread");
   static_assert((1 << LOG2N) >= P * 2 && LOG2N <= 12, "supported lengths");
@@ -267,8 +330,8 @@ __device__ __forceinline__ void fft(cd* v, cd* lds, const cd* __restrict__ tw, i
   // transforms of a kernel and keeps it live throughout, costing ~100 VGPRs
   asm volatile("" : "+v"(t));
   lds_sync();
-  const StageTw<LOG2N, P, 1> none{};
-  fft_from<LOG2N, P, INV, S, 1, PRUNED>(v, lds, tw, t, none);
+  const StageTw<LOG2N, P, 1, LT> none{};
+  fft_from<LOG2N, P, INV, S, 1, PRUNED, LT>(v, lds, tw, t, none);
 }
 
 

@@ -133,6 +133,22 @@ __device__ __forceinline__ bool col_item(int ntiles, int ngroups, int* tile, int
   return true;
 }
 
+// Stage twiddles of the column kernels' transforms come from a per-workgroup
+// LDS table (TwLds: 5.1 KiB at N = 2048 with 8 points per thread, beside the
+// 72 KiB exchange array, still two workgroups per CU).  From the global table
+// each transform would load ten 16-byte twiddles per thread, more vector-L1
+// traffic than its column.  Declares LT (true) and twp, the table the
+// transforms take, filled from the global table GTW; the fill is ordered
+// before the first transform's first twiddle read by that transform's opening
+// barrier.
+#define COL_TWIDDLES(GTW)                                                         \
+  using TL = TwLds<LOG2N, CF::P>;                                                 \
+  constexpr bool LT = true;                                                       \
+  static_assert(TL::size <= CF::NT, "one table entry per thread");                \
+  __shared__ cd twl[TL::size];                                                    \
+  const cd* twp = twl;                                                            \
+  if ((int)threadIdx.x < TL::size) twl[threadIdx.x] = TL::entry(GTW, threadIdx.x)
+
 // ---------------------------------------------------------------- column I/O
 template <int LOG2N, int P, int TR>
 __device__ __forceinline__ void col_load_rm(const cd* __restrict__ A, int ld, int j, bool ok, int t, cd* v) {
@@ -360,11 +376,12 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
   const int j = blockIdx.x * CF::C + c;
   const bool ok = j < P.Jt;
   cd* lds = lds_all + c;
+  COL_TWIDDLES(B.tw);   // as pass E's fused pass A: the same values bit for bit below 256^2
   const cd* S = blockIdx.y == 0 ? B.C1 : B.C2;
   cd* H = blockIdx.y == 0 ? B.Hc1 : B.Hc2;
   cd v[CF::P];
   col_load_trunc<LOG2N, CF::P, FILT>(S, B.filt, j, ok ? B.amask[j] : -1, t, v);
-  fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+  fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
   col_store_rm<LOG2N, CF::P, TJ>(H, P.ldJ, j, ok, t, v);
 }
 
@@ -374,20 +391,24 @@ __global__ __launch_bounds__((ColCfg<LOG2N, 16>::NT)) void k_colA(Args args) {
 // (surface_tension.c:126-127, nonlin.c:225) for the binary one.
 template <int MODEL>
 __device__ __forceinline__ cd muf_point(double c1, double c2, const KParams& P) {
+  // f(c) = c (1 - c)(1 - 2c), evaluated as c + c^2 (2c - 3)
+  auto f = [](double c) { return fma(c * c, fma(2.0, c, -3.0), c); };
   if constexpr (MODEL == MODEL_TERNARY) {
     const double c3 = (1.0 - c1 - c2);
-    const double c123 = c1 * c2 * c3;
-    const double fc1 = c1 * (1.0 - c1) * (1.0 - 2.0 * c1);
-    const double fc2 = c2 * (1.0 - c2) * (1.0 - 2.0 * c2);
-    const double fc3 = c3 * (1.0 - c3) * (1.0 - 2.0 * c3);
-    const double G1 = P.g1 * fc1 + P.X6 * c123 * c2 * c3;
-    const double G2 = P.g2 * fc2 + P.X6 * c123 * c1 * c3;
-    const double G3 = P.g3 * fc3 + P.X6 * c123 * c1 * c2;
-    const double m1 = (G1 - G2) * P.g2i + (G1 - G3) * P.g3i;
-    const double m2 = (G2 - G1) * P.g1i + (G2 - G3) * P.g3i;
-    return mk(m1 * P.fac, m2 * P.fac);
+    // X6 c1 c2 c3 times c_j c_k for G_i
+    const double c23 = c2 * c3;
+    const double q = P.X6 * (c1 * c23);
+    const double G1 = fma(P.g1, f(c1), q * c23);
+    const double G2 = fma(P.g2, f(c2), q * (c1 * c3));
+    const double G3 = fma(P.g3, f(c3), q * (c1 * c2));
+    // fac enters through the inverse coefficients (invariant across points)
+    const double g1f = P.g1i * P.fac, g2f = P.g2i * P.fac, g3f = P.g3i * P.fac;
+    const double d12 = G1 - G2;
+    const double m1 = fma(d12, g2f, (G1 - G3) * g3f);
+    const double m2 = fma(-d12, g1f, (G2 - G3) * g3f);
+    return mk(m1, m2);
   } else {
-    return mk(c1 * (1.0 - c1) * (1.0 - 2.0 * c1), 0.0);
+    return mk(f(c1), 0.0);
   }
 }
 
@@ -524,6 +545,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
   const int a = ok ? B.amask[j] : -1;
   cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
+  COL_TWIDDLES(B.tw);
   if (MODEL == MODEL_NS) g += 2;                 // NS: only the velocity groups
   if (MODEL == MODEL_BINARY && g >= 1) g += 1;   // binary: one mu group
   cd v[CF::P], s[CF::P];
@@ -531,7 +553,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
     // mu_i = T(M_i + 0.75 g_i eps k^2 c_i)  (surface_tension.c:116-128)
     // binary: mu = T(1.5 sigma eps k^2 c + 24 sigma/eps M)  (surface_tension.c:139-142,166)
     col_load_rm<LOG2N, CF::P, TF>(g == 0 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, v);
-    fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, false, CF::C, false, LT>(v, lds, twp, t);
     col_load_masked<LOG2N, CF::P>(g == 0 ? B.C1 : B.C2, j, a, t, s);
     const double mc = g == 0 ? P.mc1 : P.mc2;
 #pragma unroll
@@ -546,7 +568,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
         v[m] = mk(0.0, 0.0);
       }
     }
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 0 ? B.Hmu1 : B.Hmu2, P.ldJ, j, ok, t, v);
   } else if (g == 2) {
     // psi = w/k^2; u = i ky psi, v = -i kx psi (velocity.c:20-39)
@@ -558,17 +580,17 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
       s[m] = ks > 1e-12 ? mk(s[m].x / ks, s[m].y / ks) : mk(0.0, 0.0);
       v[m] = ctimes_ik(s[m], ky);
     }
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hu, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], -kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hv, P.ldJ, j, ok, t, v);
   } else if (CONS && g == 3) {
     // T(w) (nonlin.c:36-37 without the derivatives, which pass D's
     // conservative form takes in the products' transforms)
     col_load_trunc<LOG2N, CF::P, FILT>(B.W, B.filt, j, a, t, v);
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hw, P.ldJ, j, ok, t, v);
   } else if (g == 3) {
     // grad w: i kx w, i ky w (nonlin.c:36-37)
@@ -577,23 +599,23 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colC(Args args) {
       // T(omega) itself for the sponge term (nonlin.c:50-54)
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = s[m];
-      fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+      fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
       col_store_rm<LOG2N, CF::P, TJ>(B.Hw, P.ldJ, j, ok, t, v);
     }
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hwx, P.ldJ, j, ok, t, v);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], ky);
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(B.Hwy, P.ldJ, j, ok, t, v);
   } else {
     // d/dx c_i (nonlin.c:272,314; surface_tension.c:126,178): g = 4 c1, 5 c2
     col_load_trunc<LOG2N, CF::P, FILT>(g == 4 ? B.C1 : B.C2, B.filt, j, a, t, s);
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) v[m] = ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f));
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 4 ? B.Hdx1 : B.Hdx2, P.ldJ, j, ok, t, v);
   }
 }
@@ -1043,16 +1065,17 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
   const bool ok = j < H;
   cd* lds = lds_all + c;
   const double ky = P.f * (double)j;
+  COL_TWIDDLES(B.tw);
   cd v[CF::P], s[CF::P];
   const cd* Gm = g == 0 ? B.G1 : (g == 1 ? B.G3 : B.G4);
   col_load_rm<LOG2N, CF::P, TF>(Gm, P.ldF, j, ok, t, v);
-  fft<LOG2N, CF::P, false, CF::C>(v, lds, B.tw, t);
+  fft<LOG2N, CF::P, false, CF::C, false, LT>(v, lds, twp, t);
   if (g == 0) {
     if (MODEL != MODEL_NS || P.vpm) {
       // + i kx F(Fy): the curl of the surface-tension force (curl.c:26) and of
       // the wall penalisation (nonlin.c:76-80)
       col_load_rm<LOG2N, CF::P, TF>(B.G2, P.ldF, j, ok, t, s);
-      fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
+      fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
 #pragma unroll
       for (int m = 0; m < CF::P; ++m) v[m] = cadd(v[m], ctimes_ik(s[m], kxof<LOG2N>(t + m * T, P.f)));
     }
@@ -1065,7 +1088,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
   } else {
     // N_c = -F(u.grad c) - (lam/g) k^2 M  (nonlin.c:369-370); binary: - lam*(24 s/e k^2 M) (:240,251)
     col_load_rm<LOG2N, CF::P, TF>(g == 1 ? B.Mh1 : B.Mh2, P.ldF, j, ok, t, s);
-    fft<LOG2N, CF::P, false, CF::C>(s, lds, B.tw, t);
+    fft<LOG2N, CF::P, false, CF::C, false, LT>(s, lds, twp, t);
     const double lg = g == 1 ? P.lg1 : P.lg2;
 #pragma unroll
     for (int m = 0; m < CF::P; ++m) {
@@ -1129,7 +1152,7 @@ __global__ __launch_bounds__((ColCfg<LOG2N, PP>::NT)) void k_colE(Args args) {
       if constexpr (FILT) v[m] = okA ? filt2(s[m], filt_at<LOG2N>(B.filt, j, i)) : mk(0.0, 0.0);
       else v[m] = (okA && absidx<LOG2N>(i) <= a) ? s[m] : mk(0.0, 0.0);
     }
-    fft<LOG2N, CF::P, true, CF::C, !FILT>(v, lds, B.tw, t);
+    fft<LOG2N, CF::P, true, CF::C, !FILT, LT>(v, lds, twp, t);
     col_store_rm<LOG2N, CF::P, TJ>(g == 1 ? B.Hc1 : B.Hc2, P.ldJ, j, okA, t, v);
   }
 }
